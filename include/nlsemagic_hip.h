@@ -1,0 +1,124 @@
+/*
+ * nlsemagic_hip.h -- C ABI of the MI355X NLSE RK4 integrators
+ * (library: nlsemagic_amd/libnlsemagic_hip.so).
+ *
+ * Drop-in entry points
+ * --------------------
+ * NLSE{1,2,3}D_TAKE_STEPS_HIP[_F] take the arguments of the reference C ABI
+ * (src/NLSEmagic1D/NLSEmagic.h:1-35) in the same order and with the same
+ * meaning, and replace its CUDA entry points:
+ *   U     = split planes Uoldr/Uoldi, MATLAB column-major:
+ *           1D U[i] (i < N); 2D U[i*M + j] (N columns x M rows, M contiguous);
+ *           3D U[(i*N + j)*M + k] (an M x N x L MATLAB array)
+ *   V     = real potential, same layout
+ *   s, a  = nonlinearity and Laplacian coefficients of
+ *           i U_t + a Lap(U) + (s|U|^2 - V) U = 0  (the reference's code
+ *           convention: NLSEmagic1D_CPU.c:27-30)
+ *   h2    = h^2 (the grid step squared, not h)
+ *   BC    = 1 Dirichlet, 2 MSD, 3 Lap0, 4 one-sided (1D only); any other
+ *           value gives a zero boundary, as in the reference
+ *   chunk_size = number of RK4 steps, k = time step
+ *   method = 1 CD (2nd order), 2 2SHOC (4th order)
+ * The result is written to Unewr/Unewi.  Differences from the reference:
+ *   - the inputs are not modified (the reference CPU integrators overwrite U);
+ *   - arguments are validated; the functions return 0 on success or a
+ *     nonzero NLSE_E* code, and nlse_last_error() describes the failure
+ *     (the reference returns void and checks nothing);
+ *   - the double-precision results are bit-identical to the reference CPU
+ *     integrators (NLSEmagic{1,2}D_CPU.c, NLSE3D_TAKE_STEPS_{CD,2SHOC}.c).
+ * Device buffers are cached between calls of the same shape, so a caller
+ * looping over chunks (as the reference MATLAB drivers do) does not pay an
+ * allocation per call.  Calls are synchronous and thread-safe (serialised).
+ *
+ * Resident-state API
+ * ------------------
+ * nlse_create .. nlse_destroy keep U, V and the RK4 work arrays on the GPU
+ * across chunks (the reference re-uploads the state on every call).
+ */
+#ifndef NLSEMAGIC_HIP_H
+#define NLSEMAGIC_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NLSE_OK = 0,
+    NLSE_EINVAL = 1, /* bad argument */
+    NLSE_EHIP = 2,   /* HIP runtime error (no device, launch failure, ...) */
+    NLSE_ENOMEM = 3  /* device allocation failed */
+};
+
+/* Replaces NLSE1D_TAKE_STEPS_CUDA (src/NLSEmagic1D/NLSEmagic.h:1-2;
+ * defined at src/NLSEmagic1D/NLSEmagic1D_GPU.cu:663). */
+int NLSE1D_TAKE_STEPS_HIP(const double *Uoldr, const double *Uoldi, const double *V,
+                          double *Unewr, double *Unewi, double s, double a, double h2,
+                          int BC, int chunk_size, double k, int N, int method);
+/* Replaces NLSE1D_TAKE_STEPS_CUDA_F (NLSEmagic.h:4-5; NLSEmagic1D_GPU.cu:754). */
+int NLSE1D_TAKE_STEPS_HIP_F(const float *Uoldr, const float *Uoldi, const float *V,
+                            float *Unewr, float *Unewi, float s, float a, float h2,
+                            int BC, int chunk_size, float k, int N, int method);
+/* Replaces NLSE2D_TAKE_STEPS_CUDA (NLSEmagic.h:13-14; declared but never
+ * defined in the reference: src/NLSEmagic2D/NLSEmagic2D_GPU.cu is a copy of
+ * the 1D file).  N = columns (x), M = rows (contiguous). */
+int NLSE2D_TAKE_STEPS_HIP(const double *Uoldr, const double *Uoldi, const double *V,
+                          double *Unewr, double *Unewi, double s, double a, double h2,
+                          int BC, int chunk_size, double k, int N, int M, int method);
+/* Replaces NLSE2D_TAKE_STEPS_CUDA_F (NLSEmagic.h:16-17). */
+int NLSE2D_TAKE_STEPS_HIP_F(const float *Uoldr, const float *Uoldi, const float *V,
+                            float *Unewr, float *Unewi, float s, float a, float h2,
+                            int BC, int chunk_size, float k, int N, int M, int method);
+/* Replaces NLSE3D_TAKE_STEPS_CUDA (NLSEmagic.h:25-26; the reference ships
+ * the 3D GPU integrators only as MEX files,
+ * src/NLSEmagic3D/NLSE3D_TAKE_STEPS_{CD,2SHOC}_CUDA_D.cu). */
+int NLSE3D_TAKE_STEPS_HIP(const double *Uoldr, const double *Uoldi, const double *V,
+                          double *Unewr, double *Unewi, double s, double a, double h2,
+                          int BC, int chunk_size, double k, int N, int M, int L,
+                          int method);
+/* Replaces NLSE3D_TAKE_STEPS_CUDA_F (NLSEmagic.h:28-29). */
+int NLSE3D_TAKE_STEPS_HIP_F(const float *Uoldr, const float *Uoldi, const float *V,
+                            float *Unewr, float *Unewi, float s, float a, float h2,
+                            int BC, int chunk_size, float k, int N, int M, int L,
+                            int method);
+
+/* ---- resident-state API ---- */
+typedef struct nlse_solver nlse_solver;
+
+/* precision: 2 = double, 1 = single (the reference's `precision` codes,
+ * src/NLSEmagic1D/NLSEmagic1D.m:22-24).  dims: 1D (N), 2D (N, M), 3D (N, M, L)
+ * with unused sizes passed as 1. */
+int nlse_create(nlse_solver **out, int dim, int N, int M, int L, int method, int precision,
+                int BC, double s, double a, double h2, double k);
+void nlse_destroy(nlse_solver *h);
+/* host planes -> device state; V may be NULL to keep the previous potential */
+int nlse_set_state(nlse_solver *h, const double *Ur, const double *Ui, const double *V);
+int nlse_set_state_f(nlse_solver *h, const float *Ur, const float *Ui, const float *V);
+/* device state -> host planes (synchronises the solver's stream) */
+int nlse_get_state(nlse_solver *h, double *Ur, double *Ui);
+int nlse_get_state_f(nlse_solver *h, float *Ur, float *Ui);
+/* enqueue `steps` RK4 steps on the solver's stream (asynchronous) */
+int nlse_step(nlse_solver *h, int steps);
+/* enqueue `steps` steps between two HIP events on the solver's stream,
+ * wait, and return the elapsed device time in milliseconds */
+int nlse_step_timed(nlse_solver *h, int steps, double *ms);
+int nlse_sync(nlse_solver *h);
+/* run `steps` steps with a HIP event around every stage kernel and return
+ * the mean device time of RK4 stages 1..4 in ms_stage[0..3] (per-kernel
+ * timing for roofline reports; not available for 1D N <= 4096, which runs
+ * whole chunks in one kernel) */
+int nlse_profile_stages(nlse_solver *h, int steps, double *ms_stage);
+/* replay the chunk loop from a captured hipGraph of `steps_per_graph` steps
+ * (0 = plain stream launches, the default) */
+int nlse_set_graph(nlse_solver *h, int steps_per_graph);
+/* device bytes held by a solver of this shape (admission control) */
+long long nlse_footprint_bytes(int dim, int N, int M, int L, int method, int precision);
+
+/* description of the last error on this thread ("" if none) */
+const char *nlse_last_error(void);
+/* number of HIP devices visible (0 when no GPU) */
+int nlse_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

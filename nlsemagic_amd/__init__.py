@@ -1,0 +1,273 @@
+"""nlsemagic_amd -- MI355X-native RK4 integrators for the NLSE (NLSEmagic's hot path).
+
+Python mirror of the reference's MEX interface, calling the HIP library
+``libnlsemagic_hip.so`` through its C ABI (``include/nlsemagic_hip.h``).
+
+Array convention: a MATLAB ``M x N`` array (column-major, M contiguous) is
+the numpy array of shape ``(N, M)`` in C order; ``M x N x L`` is ``(L, N, M)``.
+1D arrays are plain vectors.  ``h2`` is h^2, as in the reference.
+
+Reference entry points mirrored here:
+  v1 MEX, 8 arguments ``U = NAME(U,V,s,a,h2,BC,chunk_size,k)``
+    (src/NLSEmagic3D/README_B.txt:11-26; called at NLSEmagic2D.m:256-279,
+    NLSEmagic3D.m:303-325): ``NLSE{1,2,3}D_TAKE_STEPS_{CD,2SHOC}_HIP_{D,F}``
+    replace the ``..._CUDA_{D,F}`` files.
+  v2 gateways, 10 arguments ``U = NAME(U,V,s,a,h2,BC,chunk_size,k,precision,method)``
+    (src/NLSEmagic1D/NLSEmagic1D_Take_Steps_GPU.c:5-90,
+    src/NLSEmagic2D/NLSEmagic2D_Take_Steps_GPU.c:5-78):
+    ``NLSEmagic{1,2,3}D_Take_Steps_GPU``.
+The result is a new complex128 array; the input is never modified.
+There is no CPU fallback: without the library or a GPU every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnlsemagic_hip.so")
+
+__all__ = [
+    "lib", "Solver", "HipError",
+    "NLSEmagic1D_Take_Steps_GPU", "NLSEmagic2D_Take_Steps_GPU", "NLSEmagic3D_Take_Steps_GPU",
+]
+
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_lib = None
+
+
+class HipError(RuntimeError):
+    """A nonzero status from the C ABI (message from nlse_last_error())."""
+
+
+def lib():
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m nlsemagic_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    for dim in (1, 2, 3):
+        nd = dim
+        fd = getattr(L, f"NLSE{dim}D_TAKE_STEPS_HIP")
+        fd.restype = C.c_int
+        fd.argtypes = [_D] * 5 + [C.c_double] * 3 + [C.c_int] * 2 + [C.c_double] + [C.c_int] * (nd + 1)
+        ff = getattr(L, f"NLSE{dim}D_TAKE_STEPS_HIP_F")
+        ff.restype = C.c_int
+        ff.argtypes = [_F] * 5 + [C.c_float] * 3 + [C.c_int] * 2 + [C.c_float] + [C.c_int] * (nd + 1)
+    L.nlse_last_error.restype = C.c_char_p
+    L.nlse_device_count.restype = C.c_int
+    L.nlse_create.restype = C.c_int
+    L.nlse_create.argtypes = [C.POINTER(C.c_void_p)] + [C.c_int] * 7 + [C.c_double] * 4
+    L.nlse_destroy.restype = None
+    L.nlse_destroy.argtypes = [C.c_void_p]
+    L.nlse_set_state.restype = C.c_int
+    L.nlse_set_state.argtypes = [C.c_void_p, _D, _D, _D]
+    L.nlse_set_state_f.restype = C.c_int
+    L.nlse_set_state_f.argtypes = [C.c_void_p, _F, _F, _F]
+    L.nlse_get_state.restype = C.c_int
+    L.nlse_get_state.argtypes = [C.c_void_p, _D, _D]
+    L.nlse_get_state_f.restype = C.c_int
+    L.nlse_get_state_f.argtypes = [C.c_void_p, _F, _F]
+    L.nlse_step.restype = C.c_int
+    L.nlse_step.argtypes = [C.c_void_p, C.c_int]
+    L.nlse_step_timed.restype = C.c_int
+    L.nlse_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.nlse_profile_stages.restype = C.c_int
+    L.nlse_profile_stages.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.nlse_sync.restype = C.c_int
+    L.nlse_sync.argtypes = [C.c_void_p]
+    L.nlse_set_graph.restype = C.c_int
+    L.nlse_set_graph.argtypes = [C.c_void_p, C.c_int]
+    L.nlse_footprint_bytes.restype = C.c_longlong
+    L.nlse_footprint_bytes.argtypes = [C.c_int] * 6
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise HipError(f"nlsemagic_hip error {rc}: {lib().nlse_last_error().decode()}")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _tail(shape):
+    """(N[, M[, L]]) integers of the C ABI for an array shape."""
+    if len(shape) == 1:
+        return (shape[0],)
+    if len(shape) == 2:
+        return (shape[0], shape[1])
+    if len(shape) == 3:
+        L, N, M = shape
+        return (N, M, L)
+    raise ValueError("U must be 1-, 2- or 3-dimensional")
+
+
+def _cplx(re_, im_):
+    out = np.empty(re_.shape, np.complex128)
+    out.real = re_
+    out.imag = im_
+    return out
+
+
+def take_steps(U, V, s, a, h2, BC, chunk_size, k, precision=2, method=1):
+    """One chunk through NLSE{1,2,3}D_TAKE_STEPS_HIP[_F].
+
+    precision 2: double throughout.  precision 1: U, V and the scalars are
+    rounded to float and the integrator runs in single precision, as the v2
+    gateway does (NLSEmagic1D_Take_Steps_GPU.c:50-79); the result is widened
+    back to double.
+    """
+    U = np.asarray(U)
+    V = np.asarray(V, dtype=np.float64)
+    if V.shape != U.shape:
+        raise ValueError(f"V shape {V.shape} differs from U shape {U.shape}")
+    dim = U.ndim
+    tail = _tail(U.shape)
+    L = lib()
+    if precision == 2:
+        ur = np.ascontiguousarray(U.real, dtype=np.float64)
+        ui = np.ascontiguousarray(U.imag if np.iscomplexobj(U) else np.zeros(U.shape), dtype=np.float64)
+        v = np.ascontiguousarray(V)
+        outr = np.empty(U.shape)
+        outi = np.empty(U.shape)
+        _check(getattr(L, f"NLSE{dim}D_TAKE_STEPS_HIP")(
+            _p(ur, C.c_double), _p(ui, C.c_double), _p(v, C.c_double), _p(outr, C.c_double),
+            _p(outi, C.c_double), float(s), float(a), float(h2), int(BC), int(chunk_size), float(k),
+            *tail, int(method)))
+        return _cplx(outr, outi)
+    if precision != 1:
+        raise ValueError("precision must be 1 (single) or 2 (double)")
+    ur = np.ascontiguousarray(U.real, dtype=np.float32)
+    ui = np.ascontiguousarray(U.imag if np.iscomplexobj(U) else np.zeros(U.shape), dtype=np.float32)
+    v = np.ascontiguousarray(V, dtype=np.float32)
+    outr = np.empty(U.shape, np.float32)
+    outi = np.empty(U.shape, np.float32)
+    f32 = lambda x: float(np.float32(x))
+    _check(getattr(L, f"NLSE{dim}D_TAKE_STEPS_HIP_F")(
+        _p(ur, C.c_float), _p(ui, C.c_float), _p(v, C.c_float), _p(outr, C.c_float), _p(outi, C.c_float),
+        f32(s), f32(a), f32(h2), int(BC), int(chunk_size), f32(k), *tail, int(method)))
+    return _cplx(outr.astype(np.float64), outi.astype(np.float64))
+
+
+def _v2(dim):
+    def gateway(U, V, s, a, h2, BC, chunk_size, k, precision, method):
+        U = np.asarray(U)
+        if U.ndim != dim:
+            raise ValueError(f"expected a {dim}D array, got shape {U.shape}")
+        return take_steps(U, V, s, a, h2, BC, chunk_size, k, precision=precision, method=method)
+
+    gateway.__name__ = f"NLSEmagic{dim}D_Take_Steps_GPU"
+    gateway.__doc__ = (f"v2 gateway U = NLSEmagic{dim}D_Take_Steps_GPU(U,V,s,a,h2,BC,chunk_size,k,precision,method)"
+                       " (src/NLSEmagic1D/NLSEmagic1D_Take_Steps_GPU.c:5-90).")
+    return gateway
+
+
+NLSEmagic1D_Take_Steps_GPU = _v2(1)
+NLSEmagic2D_Take_Steps_GPU = _v2(2)
+NLSEmagic3D_Take_Steps_GPU = _v2(3)
+
+
+def _v1(dim, method, precision):
+    def mex(U, V, s, a, h2, BC, chunk_size, k):
+        U = np.asarray(U)
+        if U.ndim != dim:
+            raise ValueError(f"expected a {dim}D array, got shape {U.shape}")
+        return take_steps(U, V, s, a, h2, BC, chunk_size, k, precision=precision, method=method)
+
+    meth = "CD" if method == 1 else "2SHOC"
+    name = f"NLSE{dim}D_TAKE_STEPS_{meth}_HIP_{'D' if precision == 2 else 'F'}"
+    mex.__name__ = name
+    mex.__doc__ = (f"v1 MEX {name}(U,V,s,a,h2,BC,chunk_size,k); replaces "
+                   f"NLSE{dim}D_TAKE_STEPS_{meth}_CUDA_{'D' if precision == 2 else 'F'}.")
+    return name, mex
+
+
+for _dim in (1, 2, 3):
+    for _m in (1, 2):
+        for _pr in (2, 1):
+            _n, _f = _v1(_dim, _m, _pr)
+            globals()[_n] = _f
+            __all__.append(_n)
+
+
+class Solver:
+    """Device-resident RK4 integrator (the nlse_create/... C ABI).
+
+    Keeps U, V and the RK4 work arrays in HBM across chunks; ``step`` is
+    asynchronous, ``state`` downloads (and synchronises).
+    """
+
+    def __init__(self, shape, method, precision, BC, s, a, h2, k):
+        dim = len(shape)
+        tail = list(_tail(tuple(shape))) + [1, 1]
+        N, M, L_ = tail[0], tail[1], tail[2]
+        self.shape = tuple(shape)
+        self.dim = dim
+        self.precision = precision
+        self._h = C.c_void_p()
+        self._lib = lib()
+        _check(self._lib.nlse_create(C.byref(self._h), dim, N, M, L_, int(method), int(precision), int(BC),
+                                     float(s), float(a), float(h2), float(k)))
+
+    def close(self):
+        if self._h:
+            self._lib.nlse_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, U, V=None):
+        U = np.asarray(U)
+        if U.shape != self.shape:
+            raise ValueError("shape mismatch")
+        ur = np.ascontiguousarray(U.real, dtype=np.float64)
+        ui = np.ascontiguousarray(U.imag if np.iscomplexobj(U) else np.zeros(U.shape), dtype=np.float64)
+        vp = None
+        if V is not None:
+            v = np.ascontiguousarray(V, dtype=np.float64)
+            if v.shape != self.shape:
+                raise ValueError("shape mismatch")
+            vp = _p(v, C.c_double)
+        _check(self._lib.nlse_set_state(self._h, _p(ur, C.c_double), _p(ui, C.c_double), vp))
+
+    def state(self):
+        outr = np.empty(self.shape)
+        outi = np.empty(self.shape)
+        _check(self._lib.nlse_get_state(self._h, _p(outr, C.c_double), _p(outi, C.c_double)))
+        return _cplx(outr, outi)
+
+    def step(self, steps):
+        _check(self._lib.nlse_step(self._h, int(steps)))
+
+    def step_timed(self, steps):
+        ms = C.c_double()
+        _check(self._lib.nlse_step_timed(self._h, int(steps), C.byref(ms)))
+        return ms.value
+
+    def profile_stages(self, steps):
+        """Mean device ms of RK4 stages 1..4 (a HIP event around each kernel)."""
+        out = (C.c_double * 4)()
+        _check(self._lib.nlse_profile_stages(self._h, int(steps), out))
+        return list(out)
+
+    def sync(self):
+        _check(self._lib.nlse_sync(self._h))
+
+    def set_graph(self, steps_per_graph):
+        _check(self._lib.nlse_set_graph(self._h, int(steps_per_graph)))
+
+
+def device_count():
+    return int(lib().nlse_device_count())

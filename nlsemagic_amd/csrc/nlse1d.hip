@@ -1,0 +1,256 @@
+// nlse1d.hip -- 1D NLSE RK4 kernels on MI355X.
+//
+// Replaces compute_F_CD / compute_F_2SHOC and the vec_add passes of
+// src/NLSEmagic1D/NLSEmagic1D_CPU.c:3-447 (single precision :449-523).
+// Two forms:
+//   - stage1d_kernel: one fused RK4 stage over 256-point tiles (any N), the
+//     stage input staged in LDS with a halo of 3 (the one-sided BC=4 formula
+//     reaches three points inward);
+//   - chunk1d_resident_kernel: a whole chunk of steps in ONE workgroup for
+//     N <= 4096; U and ktot stay in registers, the stage input and D live in
+//     LDS, and the four stages of every step are separated by barriers only.
+//     This is the launch-free form for the small 1D problems the reference
+//     targets (N = 2048): a grid-wide launch per stage would be latency bound.
+// Arithmetic follows the reference operand order with FP contraction off.
+#include "nlse_common.h"
+#include "nlse_launch.h"
+
+namespace nlse {
+
+// one-sided second difference at end b pointing inward by d (NLSEmagic1D_CPU.c:60-68)
+template <typename T, typename Get>
+__device__ __forceinline__ T os1d(Get get, int b, int d)
+{
+    return -get(b + 3 * d) + 4 * get(b + 2 * d) - 5 * get(b + d) + 2 * get(b);
+}
+
+// Evaluates D (2SHOC) and F at global points from an accessor of the stage
+// input; shared by the tiled and the resident kernels.
+template <typename T, int METHOD, typename GetY, typename GetV, typename GetD>
+struct Rhs1d {
+    GetY Y;
+    GetV Vv;
+    GetD Dd;
+    const Coefs<T>& c;
+    int N, BC;
+
+    __device__ __forceinline__ c2<T> d_int(int i) const
+    {
+        c2<T> a = Y(i + 1), m = Y(i), b = Y(i - 1);
+        return mk<T>((a.x - 2 * m.x + b.x) * c.lh2, (a.y - 2 * m.y + b.y) * c.lh2);
+    }
+    // D at any point (NLSEmagic1D_CPU.c:150-198)
+    __device__ __forceinline__ c2<T> d_at(int i) const
+    {
+        if (i > 0 && i < N - 1) return d_int(i);
+        const int q = (i == 0) ? 1 : N - 2, dir = (i == 0) ? 1 : -1;
+        switch (BC) {
+        case BC_DIRICHLET: return dir_d<T>(Y(i), Vv(i), c);
+        case BC_MSD: return msd_d<T>(d_int(q), Y(q), Vv(q), Y(i), Vv(i), c);
+        case BC_ONESIDED: {
+            auto yr = [&](int j) { return Y(j).x; };
+            auto yi = [&](int j) { return Y(j).y; };
+            return mk<T>(os1d<T>(yr, i, dir) * c.lh2, os1d<T>(yi, i, dir) * c.lh2);
+        }
+        default: return mk<T>(0, 0);
+        }
+    }
+    // F at an interior point: CD (NLSEmagic1D_CPU.c:27-30) or 2SHOC (:204-207)
+    __device__ __forceinline__ c2<T> f_int(int i) const
+    {
+        const c2<T> y = Y(i);
+        const T n = nlin<T>(c.s, y, Vv(i));
+        if (METHOD == 1) {
+            const c2<T> a = Y(i + 1), b = Y(i - 1);
+            return mk<T>(-c.ah2 * (a.y - 2 * y.y + b.y) - n * y.y, c.ah2 * (a.x - 2 * y.x + b.x) + n * y.x);
+        } else {
+            const c2<T> dc = Dd(i), da = Dd(i + 1), db = Dd(i - 1);
+            return mk<T>(c.a112 * (da.y + db.y) - c.a76 * dc.y - n * y.y,
+                         c.a76 * dc.x - c.a112 * (da.x + db.x) + n * y.x);
+        }
+    }
+    // F at any point, with the boundary rules (NLSEmagic1D_CPU.c:33-76, 210-253)
+    __device__ __forceinline__ c2<T> f_at(int i) const
+    {
+        if (i > 0 && i < N - 1) return f_int(i);
+        const int q = (i == 0) ? 1 : N - 2, dir = (i == 0) ? 1 : -1;
+        switch (BC) {
+        case BC_MSD: return msd_f<T>(f_int(q), Y(q), Y(i));
+        case BC_LAP0: return lap0_f<T>(Y(i), Vv(i), c.s);
+        case BC_ONESIDED: {
+            const c2<T> y = Y(i);
+            const T n = nlin<T>(c.s, y, Vv(i));
+            auto yr = [&](int j) { return Y(j).x; };
+            auto yi = [&](int j) { return Y(j).y; };
+            if (METHOD == 1)
+                return mk<T>(-c.ah2 * os1d<T>(yi, i, dir) - n * y.y, c.ah2 * os1d<T>(yr, i, dir) + n * y.x);
+            return mk<T>(-c.a * os1d<T>(yi, i, dir) * c.lh2 - n * y.y,
+                         c.a * os1d<T>(yr, i, dir) * c.lh2 + n * y.x);
+        }
+        default: return mk<T>(0, 0);
+        }
+    }
+};
+
+template <typename T, int METHOD, typename GY, typename GV, typename GD>
+__device__ __forceinline__ Rhs1d<T, METHOD, GY, GV, GD> make_rhs1d(GY y, GV v, GD d, const Coefs<T>& c,
+                                                                   int N, int BC)
+{
+    return Rhs1d<T, METHOD, GY, GV, GD>{y, v, d, c, N, BC};
+}
+
+// ------------------------------------------------------------- tiled stage
+
+template <typename T, int METHOD>
+__global__ __launch_bounds__(256) void stage1d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                      c2<T>* __restrict__ acc, c2<T>* yout,
+                                                      const T* __restrict__ V, Coefs<T> c, int N,
+                                                      int toff, int BC, int stage)
+{
+    constexpr int TN = 256, H = 3;
+    __shared__ c2<T> sY[TN + 2 * H];
+    __shared__ c2<T> sD[(METHOD == 2) ? TN + 2 : 1];
+    const int i0 = blockIdx.x * TN - toff;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < TN + 2 * H; e += TN) {
+        const int gi = i0 - H + e;
+        sY[e] = (gi >= 0 && gi < N) ? yin[gi] : mk<T>(0, 0);
+    }
+    __syncthreads();
+    auto Y = [&](int gi) { return sY[gi - i0 + H]; };
+    auto Vv = [&](int gi) { return V[gi]; };
+    auto Dd = [&](int gi) { return sD[gi - i0 + 1]; };
+    auto rhs = make_rhs1d<T, METHOD>(Y, Vv, Dd, c, N, BC);
+    if (METHOD == 2) {
+        for (int e = tid; e < TN + 2; e += TN) {
+            const int gi = i0 - 1 + e;
+            if (gi >= 0 && gi < N) sD[e] = rhs.d_at(gi);
+        }
+        __syncthreads();
+    }
+    const int gi = i0 + tid;
+    if (gi < 0 || gi >= N) return;
+    const c2<T> f = rhs.f_at(gi);
+    const c2<T> uu = (stage == 1) ? Y(gi) : u[gi];
+    rk4_epilogue<T>(stage, c, f, uu, acc, yout, yout, gi);
+}
+
+template <typename T>
+hipError_t launch_stage1d(const StageArgs<T>& a, hipStream_t st)
+{
+    const int toff = (a.N % 256 == 1) ? 1 : 0;
+    const int nb = (a.N + toff + 255) / 256;
+    if (a.method == 2)
+        hipLaunchKernelGGL((stage1d_kernel<T, 2>), dim3(nb), dim3(256), 0, st, a.yin, a.u, a.acc, a.yout, a.V,
+                           a.c, a.N, toff, a.BC, a.stage);
+    else
+        hipLaunchKernelGGL((stage1d_kernel<T, 1>), dim3(nb), dim3(256), 0, st, a.yin, a.u, a.acc, a.yout, a.V,
+                           a.c, a.N, toff, a.BC, a.stage);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------ whole chunk, one workgroup
+
+constexpr int RES_THREADS = 256;
+constexpr int RES_PPT = 16;  // points per thread -> N <= 4096
+
+int chunk1d_resident_max_points(int elem_bytes) { (void)elem_bytes; return RES_THREADS * RES_PPT; }
+
+template <typename T, int METHOD, int PPT>
+__global__ __launch_bounds__(RES_THREADS) void chunk1d_resident_kernel(c2<T>* u_io, const T* __restrict__ Vg,
+                                                                       Coefs<T> c, int N, int BC, int steps)
+{
+    constexpr int NMAX = RES_THREADS * PPT;
+    __shared__ c2<T> sY[NMAX];
+    __shared__ c2<T> sD[(METHOD == 2) ? NMAX : 1];
+    const int tid = threadIdx.x;
+    c2<T> U[PPT], A[PPT], Y[PPT];
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) {
+        const int i = tid + RES_THREADS * r;
+        U[r] = (i < N) ? u_io[i] : mk<T>(0, 0);
+        Y[r] = U[r];
+        A[r] = mk<T>(0, 0);
+    }
+    auto Yl = [&](int i) { return sY[i]; };
+    auto Vl = [&](int i) { return Vg[i]; };
+    auto Dl = [&](int i) { return sD[i]; };
+    auto rhs = make_rhs1d<T, METHOD>(Yl, Vl, Dl, c, N, BC);
+    for (int step = 0; step < steps; ++step) {
+#pragma unroll 1
+        for (int stage = 1; stage <= 4; ++stage) {
+#pragma unroll
+            for (int r = 0; r < PPT; ++r) {
+                const int i = tid + RES_THREADS * r;
+                if (i < N) sY[i] = Y[r];
+            }
+            __syncthreads();
+            if (METHOD == 2) {
+#pragma unroll
+                for (int r = 0; r < PPT; ++r) {
+                    const int i = tid + RES_THREADS * r;
+                    if (i < N) sD[i] = rhs.d_at(i);
+                }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int r = 0; r < PPT; ++r) {
+                const int i = tid + RES_THREADS * r;
+                if (i >= N) continue;
+                const c2<T> f = rhs.f_at(i);
+                if (stage == 1) {
+                    A[r] = f;
+                    Y[r] = mk<T>(U[r].x + c.k2 * f.x, U[r].y + c.k2 * f.y);
+                } else if (stage == 4) {
+                    A[r].x = A[r].x + (T)1.0 * f.x;
+                    A[r].y = A[r].y + (T)1.0 * f.y;
+                    U[r] = mk<T>(U[r].x + c.k6 * A[r].x, U[r].y + c.k6 * A[r].y);
+                    Y[r] = U[r];
+                } else {
+                    A[r].x = A[r].x + (T)2.0 * f.x;
+                    A[r].y = A[r].y + (T)2.0 * f.y;
+                    const T kk = (stage == 2) ? c.k2 : c.k;
+                    Y[r] = mk<T>(U[r].x + kk * f.x, U[r].y + kk * f.y);
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < PPT; ++r) {
+        const int i = tid + RES_THREADS * r;
+        if (i < N) u_io[i] = U[r];
+    }
+}
+
+template <typename T, int PPT>
+static hipError_t launch_res(c2<T>* u, const T* V, Coefs<T> c, int N, int BC, int method, int steps,
+                             hipStream_t st)
+{
+    if (method == 2)
+        hipLaunchKernelGGL((chunk1d_resident_kernel<T, 2, PPT>), dim3(1), dim3(RES_THREADS), 0, st, u, V, c, N,
+                           BC, steps);
+    else
+        hipLaunchKernelGGL((chunk1d_resident_kernel<T, 1, PPT>), dim3(1), dim3(RES_THREADS), 0, st, u, V, c, N,
+                           BC, steps);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_chunk1d_resident(c2<T>* u, const T* V, Coefs<T> c, int N, int BC, int method, int steps,
+                                   hipStream_t st)
+{
+    if (N <= RES_THREADS * 2) return launch_res<T, 2>(u, V, c, N, BC, method, steps, st);
+    if (N <= RES_THREADS * 4) return launch_res<T, 4>(u, V, c, N, BC, method, steps, st);
+    if (N <= RES_THREADS * 8) return launch_res<T, 8>(u, V, c, N, BC, method, steps, st);
+    return launch_res<T, RES_PPT>(u, V, c, N, BC, method, steps, st);
+}
+
+template hipError_t launch_stage1d<double>(const StageArgs<double>&, hipStream_t);
+template hipError_t launch_stage1d<float>(const StageArgs<float>&, hipStream_t);
+template hipError_t launch_chunk1d_resident<double>(double2*, const double*, Coefs<double>, int, int, int, int,
+                                                    hipStream_t);
+template hipError_t launch_chunk1d_resident<float>(float2*, const float*, Coefs<float>, int, int, int, int,
+                                                   hipStream_t);
+
+}  // namespace nlse

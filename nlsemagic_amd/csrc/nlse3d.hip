@@ -1,0 +1,150 @@
+// nlse3d.hip -- 3D NLSE RK4 stage kernels on MI355X.
+//
+// Replaces NLSE3D_STEP and add_cube of src/NLSEmagic3D/NLSE3D_TAKE_STEPS_CD.c
+// (:28-223) and NLSE3D_TAKE_STEPS_2SHOC.c (:28-412), with the chunk loop of
+// :308-322 / :507-521 on the host.  Element (i, j, k) is at (i*N + j)*M + k:
+// i < L slowest, k < M contiguous.
+//   CD:    one fused kernel per RK4 stage (7-point stencil + BC + update).
+//   2SHOC: per stage, a D pass (D = Lap_CD(Y)/h^2 with its boundary values,
+//          written to a scratch field) and a fused F + update pass that reads
+//          D's six neighbours and Y's twelve edge neighbours.
+// Neighbour reads go through the L1/L2 caches: a wavefront covers 64
+// consecutive k, so every plane and row neighbour is a coalesced line read.
+#include "nlse_common.h"
+#include "nlse_launch.h"
+
+namespace nlse {
+
+struct Grid3 {
+    int N, M, L;
+    long si, sj;  // strides of i and j
+};
+
+template <typename T>
+__device__ __forceinline__ c2<T> d3_int(const c2<T>* __restrict__ y, long p, const Grid3& g, const Coefs<T>& c)
+{
+    // NLSE3D_TAKE_STEPS_2SHOC.c:62-69
+    const c2<T> a = y[p + g.si], b = y[p - g.si], d = y[p + g.sj], e = y[p - g.sj], f = y[p + 1], h = y[p - 1],
+                m = y[p];
+    return mk<T>(c.lh2 * (a.x + b.x + d.x + e.x + f.x + h.x - 6 * m.x),
+                 c.lh2 * (a.y + b.y + d.y + e.y + f.y + h.y - 6 * m.y));
+}
+
+template <typename T, int METHOD>
+__device__ __forceinline__ c2<T> f3_int(const c2<T>* __restrict__ y, const c2<T>* __restrict__ D,
+                                        const T* __restrict__ V, long p, const Grid3& g, const Coefs<T>& c)
+{
+    const c2<T> m = y[p];
+    const T w = V[p] - c.s * (m.x * m.x + m.y * m.y);
+    if (METHOD == 1) {
+        // NLSE3D_TAKE_STEPS_CD.c:59-71
+        const c2<T> a = y[p + g.si], b = y[p - g.si], d = y[p + g.sj], e = y[p - g.sj], f = y[p + 1],
+                    h = y[p - 1];
+        return mk<T>(-c.ah2 * (a.y + b.y + d.y + e.y + f.y + h.y - 6 * m.y) + w * m.y,
+                     c.ah2 * (a.x + b.x + d.x + e.x + f.x + h.x - 6 * m.x) - w * m.x);
+    } else {
+        // NLSE3D_TAKE_STEPS_2SHOC.c:239-259; edge neighbours in source order
+        const long si = g.si, sj = g.sj;
+        const long e12[12] = {si + sj, sj + 1, si + 1, -si - sj, -sj - 1, -si - 1,
+                              si - sj, sj - 1, si - 1, -si + sj, -sj + 1, -si + 1};
+        c2<T> es = y[p + e12[0]];
+#pragma unroll
+        for (int t = 1; t < 12; ++t) {
+            const c2<T> q = y[p + e12[t]];
+            es.x = es.x + q.x;
+            es.y = es.y + q.y;
+        }
+        const c2<T> da = D[p + si], db = D[p - si], dd = D[p + sj], de = D[p - sj], df = D[p + 1], dh = D[p - 1],
+                    dm = D[p];
+        return mk<T>(c.a_12 * (da.y + db.y + dd.y + de.y + df.y + dh.y - 10 * dm.y) - c.a_6h2 * (es.y - 12 * m.y)
+                         + w * m.y,
+                     -c.a_12 * (da.x + db.x + dd.x + de.x + df.x + dh.x - 10 * dm.x)
+                         + c.a_6h2 * (es.x - 12 * m.x) - w * m.x);
+    }
+}
+
+__device__ __forceinline__ bool interior3(int i, int j, int k, const Grid3& g)
+{
+    return i > 0 && i < g.L - 1 && j > 0 && j < g.N - 1 && k > 0 && k < g.M - 1;
+}
+
+__device__ __forceinline__ long partner3(int i, int j, int k, const Grid3& g)
+{
+    return ((long)clamp_in(i, g.L) * g.N + clamp_in(j, g.N)) * g.M + clamp_in(k, g.M);
+}
+
+// D with its boundary values (NLSE3D_TAKE_STEPS_2SHOC.c:56-230)
+template <typename T>
+__global__ __launch_bounds__(256) void d3_kernel(const c2<T>* __restrict__ y, c2<T>* __restrict__ D,
+                                                 const T* __restrict__ V, Coefs<T> c, Grid3 g, int BC)
+{
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.z;
+    if (k >= g.M || j >= g.N) return;
+    const long p = ((long)i * g.N + j) * g.M + k;
+    c2<T> d;
+    if (interior3(i, j, k, g)) {
+        d = d3_int<T>(y, p, g, c);
+    } else if (BC == BC_DIRICHLET) {
+        d = dir_d<T>(y[p], V[p], c);
+    } else if (BC == BC_MSD) {
+        const long q = partner3(i, j, k, g);
+        d = msd_d<T>(d3_int<T>(y, q, g, c), y[q], V[q], y[p], V[p], c);
+    } else {
+        d = mk<T>(0, 0);
+    }
+    D[p] = d;
+}
+
+// F with its boundary values, then the RK4 stage update
+template <typename T, int METHOD>
+__global__ __launch_bounds__(256) void f3_kernel(const c2<T>* __restrict__ y, const c2<T>* u, c2<T>* __restrict__ acc,
+                                                 c2<T>* yout, const c2<T>* __restrict__ D, const T* __restrict__ V,
+                                                 Coefs<T> c, Grid3 g, int BC, int stage)
+{
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.z;
+    if (k >= g.M || j >= g.N) return;
+    const long p = ((long)i * g.N + j) * g.M + k;
+    c2<T> f;
+    if (interior3(i, j, k, g)) {
+        f = f3_int<T, METHOD>(y, D, V, p, g, c);
+    } else if (BC == BC_MSD) {
+        const long q = partner3(i, j, k, g);
+        f = msd_f<T>(f3_int<T, METHOD>(y, D, V, q, g, c), y[q], y[p]);
+    } else if (BC == BC_LAP0) {
+        f = lap0_f<T>(y[p], V[p], c.s);
+    } else {
+        f = mk<T>(0, 0);
+    }
+    const c2<T> uu = (stage == 1) ? y[p] : u[p];
+    rk4_epilogue<T>(stage, c, f, uu, acc, yout, yout, p);
+}
+
+template <typename T>
+hipError_t launch_stage3d(const StageArgs<T>& a, hipStream_t st)
+{
+    Grid3 g;
+    g.N = a.N;
+    g.M = a.M;
+    g.L = a.L;
+    g.sj = a.M;
+    g.si = (long)a.N * a.M;
+    dim3 grid((a.M + 63) / 64, (a.N + 3) / 4, a.L);
+    if (a.method == 2) {
+        hipLaunchKernelGGL((d3_kernel<T>), grid, dim3(256), 0, st, a.yin, a.dbuf, a.V, a.c, g, a.BC);
+        hipLaunchKernelGGL((f3_kernel<T, 2>), grid, dim3(256), 0, st, a.yin, a.u, a.acc, a.yout, a.dbuf, a.V,
+                           a.c, g, a.BC, a.stage);
+    } else {
+        hipLaunchKernelGGL((f3_kernel<T, 1>), grid, dim3(256), 0, st, a.yin, a.u, a.acc, a.yout,
+                           (const c2<T>*)nullptr, a.V, a.c, g, a.BC, a.stage);
+    }
+    return hipGetLastError();
+}
+
+template hipError_t launch_stage3d<double>(const StageArgs<double>&, hipStream_t);
+template hipError_t launch_stage3d<float>(const StageArgs<float>&, hipStream_t);
+
+}  // namespace nlse

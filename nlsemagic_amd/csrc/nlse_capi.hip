@@ -1,0 +1,531 @@
+// nlse_capi.hip -- the C ABI (include/nlsemagic_hip.h): solver objects with
+// device-resident state, the RK4 chunk loop, hipGraph replay, and the
+// NLSE{1,2,3}D_TAKE_STEPS_HIP[_F] drop-in entry points.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nlsemagic_hip.h"
+#include "nlse_launch.h"
+
+using namespace nlse;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) return fail(NLSE_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---- scalar precompute, in the precision and rounding of each reference file
+
+// NLSEmagic1D_CPU.c:384-390 (double) and :460-466 (float, all float arithmetic)
+template <typename T> void coefs1d(Coefs<T>& c, T s, T a, T h2, T k)
+{
+    c = Coefs<T>{};
+    c.s = s; c.a = a; c.k = k;
+    c.ah2 = a / h2;
+    c.k2 = k / (T)2.0;
+    c.k6 = k / (T)6.0;
+    c.l_a = (T)1.0 / a;
+    c.lh2 = (T)1.0 / h2;
+    c.a76 = a * ((T)7.0 / (T)6.0);
+    c.a112 = a * ((T)1.0 / (T)12.0);
+}
+
+// NLSEmagic2D_CPU.c:757-763 (double) and :865-871 (float: double expressions
+// rounded to float, except ah2 = a/h2, a float division)
+void coefs2d(Coefs<double>& c, double s, double a, double h2, double k)
+{
+    c = Coefs<double>{};
+    c.s = s; c.a = a; c.k = k;
+    c.l_a = 1.0 / a;
+    c.lh2 = 1.0 / h2;
+    c.k2 = k / 2.0;
+    c.k6 = k / 6.0;
+    c.a1_6h2 = a * (1.0 / (6.0 * h2));
+    c.a1_12 = a * (1.0 / 12.0);
+    c.ah2 = a / h2;
+}
+void coefs2d(Coefs<float>& c, float s, float a, float h2, float k)
+{
+    const double ad = a, h2d = h2, kd = k;
+    c = Coefs<float>{};
+    c.s = s; c.a = a; c.k = k;
+    c.l_a = (float)(1.0 / ad);
+    c.lh2 = (float)(1.0 / h2d);
+    c.k2 = (float)(kd / 2.0);
+    c.k6 = (float)(kd / 6.0);
+    c.a1_6h2 = (float)(ad * (1.0 / (6.0 * h2d)));
+    c.a1_12 = (float)(ad * (1.0 / 12.0));
+    c.ah2 = a / h2;
+}
+
+// NLSE3D_TAKE_STEPS_CD.c:304-306, NLSE3D_TAKE_STEPS_2SHOC.c:499-504 and the
+// float files' :315-317 / :509-514
+template <typename T> void coefs3d(Coefs<T>& c, T s, T a, T h2, T k)
+{
+    c = Coefs<T>{};
+    c.s = s; c.a = a; c.k = k;
+    c.ah2 = a / h2;
+    c.k2 = k / (T)2.0;
+    c.k6 = k / (T)6.0;
+    c.lh2 = (T)1.0 / h2;
+    c.a_6h2 = a / ((T)6.0 * h2);
+    c.a_12 = a / (T)12.0;
+    c.l_a = (T)1.0 / a;
+}
+
+template <typename T> __global__ void pack_kernel(const T* __restrict__ r, const T* __restrict__ i, c2<T>* __restrict__ u, long n)
+{
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) u[p] = mk<T>(r[p], i[p]);
+}
+
+template <typename T> __global__ void unpack_kernel(const c2<T>* __restrict__ u, T* __restrict__ r, T* __restrict__ i, long n)
+{
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) {
+        const c2<T> v = u[p];
+        r[p] = v.x;
+        i[p] = v.y;
+    }
+}
+
+}  // namespace
+
+struct nlse_solver {
+    int dim, method, prec, BC;
+    int N, M, L;
+    long n;
+    double s, a, h2, k;
+    Coefs<double> cd;
+    Coefs<float> cf;
+    void *U, *Y0, *Y1, *acc, *D, *V;
+    hipStream_t stream;
+    hipGraphExec_t gexec;
+    int gsteps;      // steps per graph replay (0: plain launches)
+    int gcaptured;   // steps held by gexec
+    hipEvent_t ev0, ev1;
+};
+
+namespace {
+
+size_t elem_bytes(int prec) { return prec == 2 ? 16 : 8; }
+
+int validate(int dim, int N, int M, int L, int method, int prec, int BC)
+{
+    if (dim < 1 || dim > 3) return fail(NLSE_EINVAL, "dim must be 1, 2 or 3");
+    if (method != 1 && method != 2) return fail(NLSE_EINVAL, "method must be 1 (CD) or 2 (2SHOC)");
+    if (prec != 1 && prec != 2) return fail(NLSE_EINVAL, "precision must be 1 (single) or 2 (double)");
+    if (N < 3 || (dim >= 2 && M < 3) || (dim == 3 && L < 3))
+        return fail(NLSE_EINVAL, "every grid dimension must have at least 3 points");
+    if (dim == 1 && BC == BC_ONESIDED && N < 4) return fail(NLSE_EINVAL, "BC=4 needs N >= 4");
+    const long long n = (long long)N * (dim >= 2 ? M : 1) * (dim == 3 ? L : 1);
+    if (n > (1LL << 31) - 1) return fail(NLSE_EINVAL, "grid larger than 2^31-1 points");
+    return NLSE_OK;
+}
+
+void set_params(nlse_solver* h, int BC, double s, double a, double h2, double k)
+{
+    h->BC = BC;
+    h->s = s; h->a = a; h->h2 = h2; h->k = k;
+    const float fs = (float)s, fa = (float)a, fh2 = (float)h2, fk = (float)k;
+    if (h->dim == 1) { coefs1d<double>(h->cd, s, a, h2, k); coefs1d<float>(h->cf, fs, fa, fh2, fk); }
+    else if (h->dim == 2) { coefs2d(h->cd, s, a, h2, k); coefs2d(h->cf, fs, fa, fh2, fk); }
+    else { coefs3d<double>(h->cd, s, a, h2, k); coefs3d<float>(h->cf, fs, fa, fh2, fk); }
+    if (h->gexec) {
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+        h->gcaptured = 0;
+    }
+}
+
+bool resident1d(const nlse_solver* h)
+{
+    return h->dim == 1 && h->N <= chunk1d_resident_max_points((int)elem_bytes(h->prec));
+}
+
+template <typename T>
+int enqueue_stage_t(nlse_solver* h, const Coefs<T>& c, int stage)
+{
+    using C = c2<T>;
+    C* const U = (C*)h->U;
+    C* const Y0 = (C*)h->Y0;
+    C* const Y1 = (C*)h->Y1;
+    // ping-pong of the stage input: U -> Y0 -> Y1 -> Y0 -> U
+    const C* yin[4] = {U, Y0, Y1, Y0};
+    C* yout[4] = {Y0, Y1, Y0, U};
+    StageArgs<T> a;
+    a.u = U;
+    a.acc = (C*)h->acc;
+    a.V = (const T*)h->V;
+    a.dbuf = (C*)h->D;
+    a.c = c;
+    a.N = h->N; a.M = h->M; a.L = h->L;
+    a.BC = h->BC; a.method = h->method;
+    a.stage = stage;
+    a.yin = yin[stage - 1];
+    a.yout = yout[stage - 1];
+    hipError_t e;
+    if (h->dim == 1) e = launch_stage1d<T>(a, h->stream);
+    else if (h->dim == 2) e = launch_stage2d<T>(a, h->stream);
+    else e = launch_stage3d<T>(a, h->stream);
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("stage launch: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
+
+int enqueue_stage(nlse_solver* h, int stage)
+{
+    return h->prec == 2 ? enqueue_stage_t<double>(h, h->cd, stage) : enqueue_stage_t<float>(h, h->cf, stage);
+}
+
+template <typename T>
+int enqueue_steps_t(nlse_solver* h, const Coefs<T>& c, int steps)
+{
+    using C = c2<T>;
+    if (steps <= 0) return NLSE_OK;
+    if (resident1d(h)) {
+        HIPCHK(launch_chunk1d_resident<T>((C*)h->U, (const T*)h->V, c, h->N, h->BC, h->method, steps, h->stream));
+        return NLSE_OK;
+    }
+    for (int st = 0; st < steps; ++st)
+        for (int s = 1; s <= 4; ++s) {
+            int rc = enqueue_stage_t<T>(h, c, s);
+            if (rc != NLSE_OK) return rc;
+        }
+    return NLSE_OK;
+}
+
+int enqueue_steps(nlse_solver* h, int steps)
+{
+    return h->prec == 2 ? enqueue_steps_t<double>(h, h->cd, steps) : enqueue_steps_t<float>(h, h->cf, steps);
+}
+
+int step_impl(nlse_solver* h, int steps)
+{
+    if (steps < 0) return fail(NLSE_EINVAL, "steps must be >= 0");
+    if (h->gsteps <= 0 || resident1d(h) || steps < h->gsteps) return enqueue_steps(h, steps);
+    if (!h->gexec || h->gcaptured != h->gsteps) {
+        if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_steps(h, h->gsteps);
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        if (rc != NLSE_OK) return rc;
+        if (e != hipSuccess) return fail(NLSE_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return fail(NLSE_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        h->gcaptured = h->gsteps;
+    }
+    int left = steps;
+    while (left >= h->gsteps) {
+        HIPCHK(hipGraphLaunch(h->gexec, h->stream));
+        left -= h->gsteps;
+    }
+    return enqueue_steps(h, left);
+}
+
+template <typename T>
+int upload(nlse_solver* h, const T* Ur, const T* Ui, const T* V)
+{
+    const size_t b = sizeof(T) * (size_t)h->n;
+    T* stage_r = (T*)h->Y0;
+    T* stage_i = stage_r + h->n;
+    HIPCHK(hipMemcpyAsync(stage_r, Ur, b, hipMemcpyHostToDevice, h->stream));
+    if (Ui) HIPCHK(hipMemcpyAsync(stage_i, Ui, b, hipMemcpyHostToDevice, h->stream));
+    else HIPCHK(hipMemsetAsync(stage_i, 0, b, h->stream));
+    if (V) HIPCHK(hipMemcpyAsync(h->V, V, b, hipMemcpyHostToDevice, h->stream));
+    const unsigned nb = (unsigned)((h->n + 255) / 256);
+    hipLaunchKernelGGL((pack_kernel<T>), dim3(nb), dim3(256), 0, h->stream, stage_r, stage_i, (c2<T>*)h->U, h->n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return NLSE_OK;
+}
+
+template <typename T>
+int download(nlse_solver* h, T* Ur, T* Ui)
+{
+    const size_t b = sizeof(T) * (size_t)h->n;
+    T* stage_r = (T*)h->Y0;
+    T* stage_i = stage_r + h->n;
+    const unsigned nb = (unsigned)((h->n + 255) / 256);
+    hipLaunchKernelGGL((unpack_kernel<T>), dim3(nb), dim3(256), 0, h->stream, (const c2<T>*)h->U, stage_r, stage_i,
+                       h->n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(Ur, stage_r, b, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(Ui, stage_i, b, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return NLSE_OK;
+}
+
+// ---- drop-in entry points: one cached solver per shape/precision
+
+std::mutex g_cache_mu;
+nlse_solver* g_cache[2] = {nullptr, nullptr};  // indexed by precision-1
+
+nlse_solver* cached_solver(int dim, int N, int M, int L, int method, int prec, int BC, double s, double a,
+                           double h2, double k, int* rc)
+{
+    nlse_solver*& slot = g_cache[prec - 1];
+    if (slot && (slot->dim != dim || slot->N != N || slot->M != M || slot->L != L || slot->method != method)) {
+        nlse_destroy(slot);
+        slot = nullptr;
+    }
+    if (!slot) {
+        *rc = nlse_create(&slot, dim, N, M, L, method, prec, BC, s, a, h2, k);
+        if (*rc != NLSE_OK) {
+            slot = nullptr;
+            return nullptr;
+        }
+    } else {
+        *rc = validate(dim, N, M, L, method, prec, BC);
+        if (*rc != NLSE_OK) return nullptr;
+        set_params(slot, BC, s, a, h2, k);
+    }
+    return slot;
+}
+
+template <typename T>
+int take_steps(int dim, const T* Ur, const T* Ui, const T* V, T* Unr, T* Uni, double s, double a, double h2, int BC,
+               int chunk, double k, int N, int M, int L, int method)
+{
+    if (!Ur || !V || !Unr || !Uni) return fail(NLSE_EINVAL, "null array argument");
+    if (chunk < 0) return fail(NLSE_EINVAL, "chunk_size must be >= 0");
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    int rc = NLSE_OK;
+    const int prec = sizeof(T) == 8 ? 2 : 1;
+    nlse_solver* h = cached_solver(dim, N, M, L, method, prec, BC, s, a, h2, k, &rc);
+    if (!h) return rc;
+    if ((rc = upload<T>(h, Ur, Ui, V)) != NLSE_OK) return rc;
+    if ((rc = step_impl(h, chunk)) != NLSE_OK) return rc;
+    return download<T>(h, Unr, Uni);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nlse_last_error(void) { return g_err.c_str(); }
+
+int nlse_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+long long nlse_footprint_bytes(int dim, int N, int M, int L, int method, int precision)
+{
+    const long long n = (long long)N * (dim >= 2 ? M : 1) * (dim == 3 ? L : 1);
+    const long long e = precision == 2 ? 16 : 8;
+    long long b = 4 * n * e + n * (e / 2);  // U, Y0, Y1, ktot, V
+    if (dim == 3 && method == 2) b += n * e;
+    return b;
+}
+
+int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int precision, int BC, double s,
+                double a, double h2, double k)
+{
+    if (!out) return fail(NLSE_EINVAL, "null solver pointer");
+    *out = nullptr;
+    if (dim == 1) M = L = 1;
+    if (dim == 2) L = 1;
+    int rc = validate(dim, N, M, L, method, precision, BC);
+    if (rc != NLSE_OK) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NLSE_EHIP, "no HIP device available");
+    nlse_solver* h = new nlse_solver();
+    std::memset(h, 0, sizeof(*h));
+    h->dim = dim; h->N = N; h->M = M; h->L = L;
+    h->method = method; h->prec = precision;
+    h->n = (long)N * M * L;
+    const size_t eb = elem_bytes(precision) * (size_t)h->n;
+    void** bufs[4] = {&h->U, &h->Y0, &h->Y1, &h->acc};
+    for (void** b : bufs)
+        if (hipMalloc(b, eb) != hipSuccess) { nlse_destroy(h); return fail(NLSE_ENOMEM, "hipMalloc failed"); }
+    if (hipMalloc(&h->V, eb / 2) != hipSuccess) { nlse_destroy(h); return fail(NLSE_ENOMEM, "hipMalloc failed"); }
+    if (dim == 3 && method == 2 && hipMalloc(&h->D, eb) != hipSuccess) {
+        nlse_destroy(h);
+        return fail(NLSE_ENOMEM, "hipMalloc failed");
+    }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        nlse_destroy(h);
+        return fail(NLSE_EHIP, "stream/event creation failed");
+    }
+    set_params(h, BC, s, a, h2, k);
+    // a defined initial state: zero field and potential
+    (void)hipMemsetAsync(h->U, 0, eb, h->stream);
+    (void)hipMemsetAsync(h->V, 0, eb / 2, h->stream);
+    (void)hipStreamSynchronize(h->stream);
+    *out = h;
+    return NLSE_OK;
+}
+
+void nlse_destroy(nlse_solver* h)
+{
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    void* bufs[6] = {h->U, h->Y0, h->Y1, h->acc, h->D, h->V};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int nlse_set_state(nlse_solver* h, const double* Ur, const double* Ui, const double* V)
+{
+    if (!h || !Ur) return fail(NLSE_EINVAL, "null argument");
+    if (h->prec == 2) return upload<double>(h, Ur, Ui, V);
+    std::vector<float> r(h->n), i(h->n, 0.0f), v;
+    for (long p = 0; p < h->n; ++p) r[p] = (float)Ur[p];
+    if (Ui) for (long p = 0; p < h->n; ++p) i[p] = (float)Ui[p];
+    if (V) { v.resize(h->n); for (long p = 0; p < h->n; ++p) v[p] = (float)V[p]; }
+    return upload<float>(h, r.data(), i.data(), V ? v.data() : nullptr);
+}
+
+int nlse_set_state_f(nlse_solver* h, const float* Ur, const float* Ui, const float* V)
+{
+    if (!h || !Ur) return fail(NLSE_EINVAL, "null argument");
+    if (h->prec != 1) return fail(NLSE_EINVAL, "nlse_set_state_f on a double-precision solver");
+    return upload<float>(h, Ur, Ui, V);
+}
+
+int nlse_get_state(nlse_solver* h, double* Ur, double* Ui)
+{
+    if (!h || !Ur || !Ui) return fail(NLSE_EINVAL, "null argument");
+    if (h->prec == 2) return download<double>(h, Ur, Ui);
+    std::vector<float> r(h->n), i(h->n);
+    int rc = download<float>(h, r.data(), i.data());
+    if (rc != NLSE_OK) return rc;
+    for (long p = 0; p < h->n; ++p) { Ur[p] = r[p]; Ui[p] = i[p]; }
+    return NLSE_OK;
+}
+
+int nlse_get_state_f(nlse_solver* h, float* Ur, float* Ui)
+{
+    if (!h || !Ur || !Ui) return fail(NLSE_EINVAL, "null argument");
+    if (h->prec != 1) return fail(NLSE_EINVAL, "nlse_get_state_f on a double-precision solver");
+    return download<float>(h, Ur, Ui);
+}
+
+int nlse_step(nlse_solver* h, int steps)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    return step_impl(h, steps);
+}
+
+int nlse_step_timed(nlse_solver* h, int steps, double* ms)
+{
+    if (!h || !ms) return fail(NLSE_EINVAL, "null argument");
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    int rc = step_impl(h, steps);
+    if (rc != NLSE_OK) return rc;
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, h->ev0, h->ev1));
+    *ms = f;
+    return NLSE_OK;
+}
+
+int nlse_profile_stages(nlse_solver* h, int steps, double* ms_stage)
+{
+    if (!h || !ms_stage || steps <= 0) return fail(NLSE_EINVAL, "bad argument");
+    if (resident1d(h)) return fail(NLSE_EINVAL, "small 1D problems run whole chunks in one kernel");
+    hipEvent_t ev[5];
+    for (int i = 0; i < 5; ++i) HIPCHK(hipEventCreate(&ev[i]));
+    double acc[4] = {0, 0, 0, 0};
+    int rc = NLSE_OK;
+    for (int st = 0; st < steps && rc == NLSE_OK; ++st) {
+        if (hipEventRecord(ev[0], h->stream) != hipSuccess) { rc = fail(NLSE_EHIP, "hipEventRecord"); break; }
+        for (int s = 1; s <= 4 && rc == NLSE_OK; ++s) {
+            rc = enqueue_stage(h, s);
+            if (rc == NLSE_OK && hipEventRecord(ev[s], h->stream) != hipSuccess) rc = fail(NLSE_EHIP, "hipEventRecord");
+        }
+        if (rc != NLSE_OK) break;
+        if (hipEventSynchronize(ev[4]) != hipSuccess) { rc = fail(NLSE_EHIP, "hipEventSynchronize"); break; }
+        for (int s = 0; s < 4; ++s) {
+            float f = 0.f;
+            (void)hipEventElapsedTime(&f, ev[s], ev[s + 1]);
+            acc[s] += f;
+        }
+    }
+    for (int i = 0; i < 5; ++i) (void)hipEventDestroy(ev[i]);
+    if (rc != NLSE_OK) return rc;
+    for (int s = 0; s < 4; ++s) ms_stage[s] = acc[s] / steps;
+    return NLSE_OK;
+}
+
+int nlse_sync(nlse_solver* h)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return NLSE_OK;
+}
+
+int nlse_set_graph(nlse_solver* h, int steps_per_graph)
+{
+    if (!h || steps_per_graph < 0) return fail(NLSE_EINVAL, "bad argument");
+    h->gsteps = steps_per_graph;
+    return NLSE_OK;
+}
+
+int NLSE1D_TAKE_STEPS_HIP(const double* Uoldr, const double* Uoldi, const double* V, double* Unewr, double* Unewi,
+                          double s, double a, double h2, int BC, int chunk_size, double k, int N, int method)
+{
+    return take_steps<double>(1, Uoldr, Uoldi, V, Unewr, Unewi, s, a, h2, BC, chunk_size, k, N, 1, 1, method);
+}
+
+int NLSE1D_TAKE_STEPS_HIP_F(const float* Uoldr, const float* Uoldi, const float* V, float* Unewr, float* Unewi,
+                            float s, float a, float h2, int BC, int chunk_size, float k, int N, int method)
+{
+    return take_steps<float>(1, Uoldr, Uoldi, V, Unewr, Unewi, s, a, h2, BC, chunk_size, k, N, 1, 1, method);
+}
+
+int NLSE2D_TAKE_STEPS_HIP(const double* Uoldr, const double* Uoldi, const double* V, double* Unewr, double* Unewi,
+                          double s, double a, double h2, int BC, int chunk_size, double k, int N, int M, int method)
+{
+    return take_steps<double>(2, Uoldr, Uoldi, V, Unewr, Unewi, s, a, h2, BC, chunk_size, k, N, M, 1, method);
+}
+
+int NLSE2D_TAKE_STEPS_HIP_F(const float* Uoldr, const float* Uoldi, const float* V, float* Unewr, float* Unewi,
+                            float s, float a, float h2, int BC, int chunk_size, float k, int N, int M, int method)
+{
+    return take_steps<float>(2, Uoldr, Uoldi, V, Unewr, Unewi, s, a, h2, BC, chunk_size, k, N, M, 1, method);
+}
+
+int NLSE3D_TAKE_STEPS_HIP(const double* Uoldr, const double* Uoldi, const double* V, double* Unewr, double* Unewi,
+                          double s, double a, double h2, int BC, int chunk_size, double k, int N, int M, int L,
+                          int method)
+{
+    return take_steps<double>(3, Uoldr, Uoldi, V, Unewr, Unewi, s, a, h2, BC, chunk_size, k, N, M, L, method);
+}
+
+int NLSE3D_TAKE_STEPS_HIP_F(const float* Uoldr, const float* Uoldi, const float* V, float* Unewr, float* Unewi,
+                            float s, float a, float h2, int BC, int chunk_size, float k, int N, int M, int L,
+                            int method)
+{
+    return take_steps<float>(3, Uoldr, Uoldi, V, Unewr, Unewi, s, a, h2, BC, chunk_size, k, N, M, L, method);
+}
+
+}  // extern "C"

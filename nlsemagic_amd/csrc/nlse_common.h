@@ -1,0 +1,118 @@
+// nlse_common.h -- shared device-side definitions for the NLSE RK4 kernels.
+//
+// All kernels are compiled with -ffp-contract=off: every multiply and add
+// rounds separately, in the operand order of the reference CPU integrators
+// (src/NLSEmagic{1,2}D/NLSEmagic{1,2}D_CPU.c, src/NLSEmagic3D/NLSE3D_TAKE_STEPS_*.c),
+// so the double-precision results are bit-identical to them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nlse {
+
+// complex value stored interleaved (real, imaginary) in HBM and LDS
+template <typename T> struct C2;
+template <> struct C2<double> { using type = double2; };
+template <> struct C2<float> { using type = float2; };
+template <typename T> using c2 = typename C2<T>::type;
+
+template <typename T> __host__ __device__ inline c2<T> mk(T x, T y) { c2<T> r; r.x = x; r.y = y; return r; }
+
+// Scalars of one integrator call, precomputed on the host exactly as the
+// reference file of each dimension does (see coefs_* in nlse_capi.cpp).
+template <typename T> struct Coefs {
+    T s, a, k, k2, k6;
+    T ah2;            // a/h^2 (CD)
+    T lh2, l_a;       // 1/h^2, 1/a (2SHOC)
+    T a76, a112;      // 1D 2SHOC
+    T a1_6h2, a1_12;  // 2D 2SHOC
+    T a_6h2, a_12;    // 3D 2SHOC
+};
+
+// boundary-condition codes of the reference (NLSE3D_TAKE_STEPS_CD.c:18,
+// NLSEmagic1D_CPU.c:33-76); any other value behaves like a zero boundary
+enum { BC_DIRICHLET = 1, BC_MSD = 2, BC_LAP0 = 3, BC_ONESIDED = 4 };
+
+// RK4 stage index -> epilogue (NLSEmagic1D_CPU.c:405-418)
+//  1: ktot = K;          Y = U + k/2 K
+//  2: ktot = ktot + 2K;  Y = U + k/2 K
+//  3: ktot = ktot + 2K;  Y = U + k K
+//  4: ktot = ktot + 1K;  U = U + k/6 ktot
+template <typename T>
+__device__ __forceinline__ void rk4_epilogue(int stage, const Coefs<T>& c, c2<T> f, c2<T> u,
+                                             c2<T>* __restrict__ acc, c2<T>* __restrict__ yout,
+                                             c2<T>* __restrict__ uout, long p)
+{
+    if (stage == 1) {
+        acc[p] = f;
+        yout[p] = mk<T>(u.x + c.k2 * f.x, u.y + c.k2 * f.y);
+    } else if (stage == 4) {
+        c2<T> a = acc[p];
+        a.x = a.x + (T)1.0 * f.x;
+        a.y = a.y + (T)1.0 * f.y;
+        uout[p] = mk<T>(u.x + c.k6 * a.x, u.y + c.k6 * a.y);
+    } else {
+        c2<T> a = acc[p];
+        a.x = a.x + (T)2.0 * f.x;
+        a.y = a.y + (T)2.0 * f.y;
+        acc[p] = a;
+        const T kk = (stage == 2) ? c.k2 : c.k;
+        yout[p] = mk<T>(u.x + kk * f.x, u.y + kk * f.y);
+    }
+}
+
+// s|U|^2 - V: the bracket of "- (s*(Ur*Ur + Ui*Ui) - V)*Ui"
+template <typename T> __device__ __forceinline__ T nlin(T s, c2<T> u, T v)
+{
+    return s * (u.x * u.x + u.y * u.y) - v;
+}
+
+// MSD: boundary F from the interior partner's F and U (NLSEmagic2D_CPU.c:64-68)
+template <typename T> __device__ __forceinline__ c2<T> msd_f(c2<T> fq, c2<T> uq, c2<T> ub)
+{
+    T om = (fq.y * uq.x - fq.x * uq.y) / (uq.x * uq.x + uq.y * uq.y);
+    return mk<T>(-om * ub.y, om * ub.x);
+}
+
+// MSD: boundary D from the partner's D (NLSEmagic2D_CPU.c:297-303)
+template <typename T>
+__device__ __forceinline__ c2<T> msd_d(c2<T> dq, c2<T> uq, T vq, c2<T> ub, T vb, const Coefs<T>& c)
+{
+    T A = (dq.x * uq.x + dq.y * uq.y) / (uq.x * uq.x + uq.y * uq.y);
+    T nb = nlin<T>(c.s, ub, vb);
+    T nb1 = nlin<T>(c.s, uq, vq);
+    return mk<T>((A + c.l_a * (nb1 - nb)) * ub.x, (A + c.l_a * (nb1 - nb)) * ub.y);
+}
+
+// Lap0 boundary F (NLSEmagic2D_CPU.c:109-112)
+template <typename T> __device__ __forceinline__ c2<T> lap0_f(c2<T> ub, T vb, T s)
+{
+    T n = nlin<T>(s, ub, vb);
+    return mk<T>(-n * ub.y, n * ub.x);
+}
+
+// Dirichlet boundary D for 2SHOC (NLSEmagic2D_CPU.c:282-285)
+template <typename T> __device__ __forceinline__ c2<T> dir_d(c2<T> ub, T vb, const Coefs<T>& c)
+{
+    T t = c.l_a * (vb - c.s * (ub.x * ub.x + ub.y * ub.y));
+    return mk<T>(t * ub.x, t * ub.y);
+}
+
+__device__ __forceinline__ int clamp_in(int x, int n) { return x < 1 ? 1 : (x > n - 2 ? n - 2 : x); }
+
+// Launch-time description of a tiled 2D or 3D sweep.  Tiles are numbered so
+// that consecutive tiles in space land on the same XCD (workgroups are dealt
+// round-robin to the 8 XCDs; each XCD has its own L2), which keeps the halo
+// re-reads of neighbouring tiles inside one L2.
+struct TileMap {
+    int ntiles;   // tiles in the sweep
+    int per_xcd;  // ceil(ntiles / 8)
+    __device__ __forceinline__ int tile() const
+    {
+        int b = blockIdx.x;
+        return (b & 7) * per_xcd + (b >> 3);
+    }
+};
+
+}  // namespace nlse

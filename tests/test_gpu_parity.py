@@ -1,0 +1,139 @@
+"""HIP integrators vs the CPU oracle and the reference's golden vectors.
+
+Every call goes through the C ABI (NLSE{1,2,3}D_TAKE_STEPS_HIP[_F]).  The bar
+is bit-exact equality: the kernels keep the reference's operand order with FP
+contraction off, so double results match the reference CPU integrators bit
+for bit, and single-precision results match the reference's single-precision
+integrators bit for bit.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal, golden_cases, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    import nlsemagic_amd as nm
+    if nm.device_count() < 1:
+        pytest.fail("no HIP device visible")
+    return nm
+
+
+def _announce(*what):
+    print("CASE", *what, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("case", golden_cases(), ids=lambda c: c[0]["name"])
+def test_hip_matches_golden(hip, case):
+    c, U, V, expected = case
+    _announce(c["name"])
+    got = hip.take_steps(U, V, c["s"], c["a"], c["h2"], c["BC"], c["chunk_size"], c["k"],
+                         precision=c["precision"], method=c["method"])
+    assert bits_equal(got, expected), c["name"]
+
+
+def _cases():
+    out = []
+    seed = 1000
+    shapes = {1: [(4,), (7,), (300,), (4096,), (4097,), (5000,), (513,)],
+              2: [(3, 3), (17, 65), (33, 130), (50, 64), (18, 7)],
+              3: [(3, 3, 3), (5, 7, 66), (9, 6, 65), (4, 17, 10)]}
+    bcs = {1: [1, 2, 3, 4, 0], 2: [1, 2, 3, 0], 3: [1, 2, 3, 0]}
+    for dim in (1, 2, 3):
+        for shape in shapes[dim]:
+            for method in (1, 2):
+                for prec in (2, 1):
+                    for BC in bcs[dim]:
+                        seed += 1
+                        out.append((dim, shape, method, prec, BC, seed))
+    return out
+
+
+@pytest.mark.parametrize("dim,shape,method,prec,BC,seed", _cases())
+def test_hip_matches_oracle(hip, oracle, dim, shape, method, prec, BC, seed):
+    _announce(dim, shape, method, prec, BC, seed)
+    rng = np.random.default_rng(seed)
+    U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
+    V = 0.5 * rng.uniform(-1, 1, shape)
+    h2 = 0.04
+    k = 0.1 * h2
+    s = -1.0 if BC != 3 else 0.8
+    chunk = 4
+    exp = oracle.take_steps(U, V, s, 0.9, h2, BC, chunk, k, method, precision=prec)
+    got = hip.take_steps(U, V, s, 0.9, h2, BC, chunk, k, precision=prec, method=method)
+    assert np.all(np.isfinite(exp))
+    assert bits_equal(got, exp), f"max |diff| {np.max(np.abs(got - exp))}"
+
+
+def test_hip_does_not_modify_inputs(hip):
+    rng = np.random.default_rng(5)
+    U = rng.uniform(-1, 1, (20, 30)) + 1j * rng.uniform(-1, 1, (20, 30))
+    V = rng.uniform(-1, 1, (20, 30))
+    U0, V0 = U.copy(), V.copy()
+    hip.take_steps(U, V, -1.0, 1.0, 0.01, 2, 3, 0.001, precision=2, method=2)
+    assert np.array_equal(U, U0) and np.array_equal(V, V0)
+
+
+def test_real_input_gets_zero_imaginary_part(hip, oracle):
+    """A real U is integrated as U + 0i (NLSEmagic2D_Take_Steps_GPU.c:20-28)."""
+    rng = np.random.default_rng(6)
+    U = 1 + 0.2 * rng.uniform(-1, 1, (16, 16))
+    V = np.zeros((16, 16))
+    got = hip.NLSEmagic2D_Take_Steps_GPU(U, V, -1.0, 1.0, 0.01, 1, 5, 0.002, 2, 1)
+    exp = oracle.take_steps(U.astype(complex), V, -1.0, 1.0, 0.01, 1, 5, 0.002, 1)
+    assert bits_equal(got, exp)
+
+
+@pytest.mark.parametrize("graph", [0, 3])
+def test_resident_solver_chunks_equal_one_call(hip, oracle, graph):
+    """State kept on the GPU across chunks = one long call (2D 2SHOC, MSD),
+    with and without hipGraph replay of the chunk loop."""
+    rng = np.random.default_rng(8)
+    shape = (40, 70)
+    U = 1 + 0.1 * rng.uniform(-1, 1, shape) + 0.1j * rng.uniform(-1, 1, shape)
+    V = 0.1 * rng.uniform(-1, 1, shape)
+    s, a, h2, k = -1.0, 1.0, 0.01, 0.002
+    sol = hip.Solver(shape, 2, 2, 2, s, a, h2, k)
+    sol.set_graph(graph)
+    sol.set_state(U, V)
+    for _ in range(3):
+        sol.step(7)
+    got = sol.state()
+    sol.close()
+    exp = oracle.take_steps(U, V, s, a, h2, 2, 21, k, 2)
+    assert bits_equal(got, exp)
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_full_size_2d_bitwise_short_run(hip, oracle, method):
+    """1024 x 1024 dark vortex, MSD, a few steps: bit-exact at full size."""
+    n = 1024
+    h = 0.1
+    x = -51.2 + h * np.arange(n)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    r = np.hypot(X, Y)
+    rng = np.random.default_rng(0)
+    U = np.tanh(r) * np.exp(1j * np.arctan2(Y, X)) + 1e-3 * (rng.uniform(-1, 1, r.shape) + 1j * rng.uniform(-1, 1, r.shape))
+    V = np.zeros(r.shape)
+    k = (0.3 if method == 1 else 0.2121) * h * h
+    _announce("full2d", method)
+    got = hip.take_steps(U, V, -1.0, 1.0, h * h, 2, 3, k, precision=2, method=method)
+    exp = oracle.take_steps(U, V, -1.0, 1.0, h * h, 2, 3, k, method)
+    assert bits_equal(got, exp), rel_l2(got, exp)
+
+
+def test_invalid_arguments_raise(hip):
+    U = np.ones((8, 8), complex)
+    V = np.zeros((8, 8))
+    with pytest.raises(hip.HipError):
+        hip.take_steps(U, V, -1, 1, 0.01, 2, 1, 0.001, precision=2, method=3)
+    with pytest.raises(hip.HipError):
+        hip.take_steps(np.ones((2, 8), complex), np.zeros((2, 8)), -1, 1, 0.01, 2, 1, 0.001, precision=2, method=1)
