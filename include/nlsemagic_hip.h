@@ -102,16 +102,31 @@ int nlse_step(nlse_solver *h, int steps);
  * wait, and return the elapsed device time in milliseconds */
 int nlse_step_timed(nlse_solver *h, int steps, double *ms);
 int nlse_sync(nlse_solver *h);
-/* run `steps` steps with a HIP event around every stage kernel and return
- * the mean device time of RK4 stages 1..4 in ms_stage[0..3] (per-kernel
- * timing for roofline reports; not available for 1D N <= 4096, which runs
- * whole chunks in one kernel) */
-int nlse_profile_stages(nlse_solver *h, int steps, double *ms_stage);
+/* run `steps` steps with a HIP event around every kernel launch and return
+ * the mean device time of each launch of a step in ms_launch[0..nlaunch-1]
+ * (nlaunch = 4 / stages per launch; per-kernel timing for roofline reports;
+ * not available for 1D N <= 4096, which runs whole chunks in one kernel) */
+int nlse_profile_launches(nlse_solver *h, int steps, double *ms_launch, int *nlaunch);
+/* 2D: RK4 stages fused per kernel launch, 1, 2 or 4 (default 2; the results
+ * are identical for every choice) */
+int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
+/* 2D: tile height of the stage kernels, 8 (default), 16 or 32 rows of 64
+ * columns (results identical) */
+int nlse_set_tile(nlse_solver *h, int tile_rows);
+/* 2D: time `steps` steps for every (stages per launch, tile height) pair on a
+ * saved copy of the state, keep the fastest, and restore the state; reports
+ * the choice and its ms per step (any pointer may be NULL).  1D/3D: no-op. */
+int nlse_autotune(nlse_solver *h, int steps, int *best_fusion, int *best_tile, double *best_ms);
 /* replay the chunk loop from a captured hipGraph of `steps_per_graph` steps
  * (0 = plain stream launches, the default) */
 int nlse_set_graph(nlse_solver *h, int steps_per_graph);
 /* device bytes held by a solver of this shape (admission control) */
 long long nlse_footprint_bytes(int dim, int N, int M, int L, int method, int precision);
+
+/* measurement helper: mean device ms of a plain streaming copy of `bytes`
+ * (multiple of 16) between two fresh device buffers, over `reps` launches;
+ * gives the HBM counters' reading for a known byte count */
+int nlse_copy_probe(long long bytes, int reps, double *ms);
 
 /* description of the last error on this thread ("" if none) */
 const char *nlse_last_error(void);

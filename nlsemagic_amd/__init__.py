@@ -77,12 +77,21 @@ def lib():
     L.nlse_step.argtypes = [C.c_void_p, C.c_int]
     L.nlse_step_timed.restype = C.c_int
     L.nlse_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
-    L.nlse_profile_stages.restype = C.c_int
-    L.nlse_profile_stages.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.nlse_profile_launches.restype = C.c_int
+    L.nlse_profile_launches.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.nlse_autotune.restype = C.c_int
+    L.nlse_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_double)]
+    L.nlse_set_tile.restype = C.c_int
+    L.nlse_set_tile.argtypes = [C.c_void_p, C.c_int]
+    L.nlse_set_fusion.restype = C.c_int
+    L.nlse_set_fusion.argtypes = [C.c_void_p, C.c_int]
     L.nlse_sync.restype = C.c_int
     L.nlse_sync.argtypes = [C.c_void_p]
     L.nlse_set_graph.restype = C.c_int
     L.nlse_set_graph.argtypes = [C.c_void_p, C.c_int]
+    L.nlse_copy_probe.restype = C.c_int
+    L.nlse_copy_probe.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.nlse_footprint_bytes.restype = C.c_longlong
     L.nlse_footprint_bytes.argtypes = [C.c_int] * 6
     _lib = L
@@ -256,17 +265,42 @@ class Solver:
         _check(self._lib.nlse_step_timed(self._h, int(steps), C.byref(ms)))
         return ms.value
 
-    def profile_stages(self, steps):
-        """Mean device ms of RK4 stages 1..4 (a HIP event around each kernel)."""
+    def profile_launches(self, steps):
+        """Mean device ms of each kernel launch of an RK4 step (HIP events)."""
         out = (C.c_double * 4)()
-        _check(self._lib.nlse_profile_stages(self._h, int(steps), out))
-        return list(out)
+        n = C.c_int()
+        _check(self._lib.nlse_profile_launches(self._h, int(steps), out, C.byref(n)))
+        return list(out)[: n.value]
+
+    def autotune(self, steps=5):
+        """Pick the fastest (stages per launch, tile height) for this problem.
+
+        Returns (stages_per_launch, tile_rows, ms_per_step); the state is
+        unchanged."""
+        f, t, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(self._lib.nlse_autotune(self._h, int(steps), C.byref(f), C.byref(t), C.byref(ms)))
+        return f.value, t.value, ms.value
+
+    def set_tile(self, tile_rows):
+        """2D: tile height of the stage kernels (8, 16 or 32 rows)."""
+        _check(self._lib.nlse_set_tile(self._h, int(tile_rows)))
+
+    def set_fusion(self, stages_per_launch):
+        """2D: RK4 stages per kernel launch (1, 2 or 4)."""
+        _check(self._lib.nlse_set_fusion(self._h, int(stages_per_launch)))
 
     def sync(self):
         _check(self._lib.nlse_sync(self._h))
 
     def set_graph(self, steps_per_graph):
         _check(self._lib.nlse_set_graph(self._h, int(steps_per_graph)))
+
+
+def copy_probe(nbytes, reps=10):
+    """Mean ms of a plain device copy of nbytes (HBM counter calibration)."""
+    ms = C.c_double()
+    _check(lib().nlse_copy_probe(int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def device_count():

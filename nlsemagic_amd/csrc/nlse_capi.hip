@@ -105,6 +105,13 @@ template <typename T> __global__ void unpack_kernel(const c2<T>* __restrict__ u,
     }
 }
 
+// plain streaming copy (16 B per lane), used to calibrate HBM byte counters
+__global__ __launch_bounds__(256) void copy_probe_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                         long n)
+{
+    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256) dst[p] = src[p];
+}
+
 }  // namespace
 
 struct nlse_solver {
@@ -116,9 +123,12 @@ struct nlse_solver {
     Coefs<float> cf;
     void *U, *Y0, *Y1, *acc, *D, *V;
     hipStream_t stream;
-    hipGraphExec_t gexec;
-    int gsteps;      // steps per graph replay (0: plain launches)
-    int gcaptured;   // steps held by gexec
+    hipGraphExec_t gexec[2];  // per buffer holding U when the graph starts
+    int gsteps;               // steps per graph replay (0: plain launches)
+    int gcaptured[2];         // steps held by gexec[i]
+    int fuse;                 // 2D: RK4 stages per launch (1, 2 or 4)
+    int tile;                 // 2D: tile height (8, 16, 32)
+    void* ubase;              // the buffer U was allocated in (U swaps with Y1 when fuse > 1)
     hipEvent_t ev0, ev1;
 };
 
@@ -139,6 +149,18 @@ int validate(int dim, int N, int M, int L, int method, int prec, int BC)
     return NLSE_OK;
 }
 
+void drop_graphs(nlse_solver* h)
+{
+    for (int i = 0; i < 2; ++i) {
+        if (h->gexec[i]) (void)hipGraphExecDestroy(h->gexec[i]);
+        h->gexec[i] = nullptr;
+        h->gcaptured[i] = 0;
+    }
+}
+
+// RK4 stages per launch actually used (temporal fusion is a 2D feature)
+int fusion(const nlse_solver* h) { return h->dim == 2 ? h->fuse : 1; }
+
 void set_params(nlse_solver* h, int BC, double s, double a, double h2, double k)
 {
     h->BC = BC;
@@ -147,11 +169,7 @@ void set_params(nlse_solver* h, int BC, double s, double a, double h2, double k)
     if (h->dim == 1) { coefs1d<double>(h->cd, s, a, h2, k); coefs1d<float>(h->cf, fs, fa, fh2, fk); }
     else if (h->dim == 2) { coefs2d(h->cd, s, a, h2, k); coefs2d(h->cf, fs, fa, fh2, fk); }
     else { coefs3d<double>(h->cd, s, a, h2, k); coefs3d<float>(h->cf, fs, fa, fh2, fk); }
-    if (h->gexec) {
-        (void)hipGraphExecDestroy(h->gexec);
-        h->gexec = nullptr;
-        h->gcaptured = 0;
-    }
+    drop_graphs(h);
 }
 
 bool resident1d(const nlse_solver* h)
@@ -177,6 +195,7 @@ int enqueue_stage_t(nlse_solver* h, const Coefs<T>& c, int stage)
     a.c = c;
     a.N = h->N; a.M = h->M; a.L = h->L;
     a.BC = h->BC; a.method = h->method;
+    a.tile = h->tile;
     a.stage = stage;
     a.yin = yin[stage - 1];
     a.yout = yout[stage - 1];
@@ -193,6 +212,52 @@ int enqueue_stage(nlse_solver* h, int stage)
     return h->prec == 2 ? enqueue_stage_t<double>(h, h->cd, stage) : enqueue_stage_t<float>(h, h->cf, stage);
 }
 
+// launch l (0 <= l < 4/fusion) of one RK4 step; a fused step leaves the new
+// state in Y1 and swaps it with U
+template <typename T>
+int enqueue_launch_t(nlse_solver* h, const Coefs<T>& c, int l)
+{
+    using C = c2<T>;
+    const int fz = fusion(h);
+    if (fz == 1) return enqueue_stage_t<T>(h, c, l + 1);
+    StageArgs<T> a;
+    a.acc = (C*)h->acc;
+    a.V = (const T*)h->V;
+    a.dbuf = nullptr;
+    a.c = c;
+    a.N = h->N; a.M = h->M; a.L = h->L;
+    a.BC = h->BC; a.method = h->method;
+    a.tile = h->tile;
+    a.stage = 0;
+    a.u = (const C*)h->U;
+    hipError_t e;
+    bool last;
+    if (fz == 2 && l == 0) {
+        a.yin = (const C*)h->U;
+        a.yout = (C*)h->Y0;  // Y_2 (and ktot) after stages 1-2
+        e = launch_pair2d<T>(a, true, h->stream);
+        last = false;
+    } else if (fz == 2) {
+        a.yin = (const C*)h->Y0;
+        a.yout = (C*)h->Y1;
+        e = launch_pair2d<T>(a, false, h->stream);
+        last = true;
+    } else {
+        a.yin = (const C*)h->U;
+        a.yout = (C*)h->Y1;
+        e = launch_step2d<T>(a, h->stream);
+        last = true;
+    }
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("fused launch: ") + hipGetErrorString(e));
+    if (last) std::swap(h->U, h->Y1);
+    return NLSE_OK;
+}
+
+int enqueue_launch(nlse_solver* h, int l)
+{
+    return h->prec == 2 ? enqueue_launch_t<double>(h, h->cd, l) : enqueue_launch_t<float>(h, h->cf, l);
+}
+
 template <typename T>
 int enqueue_steps_t(nlse_solver* h, const Coefs<T>& c, int steps)
 {
@@ -202,9 +267,10 @@ int enqueue_steps_t(nlse_solver* h, const Coefs<T>& c, int steps)
         HIPCHK(launch_chunk1d_resident<T>((C*)h->U, (const T*)h->V, c, h->N, h->BC, h->method, steps, h->stream));
         return NLSE_OK;
     }
+    const int nl = 4 / fusion(h);
     for (int st = 0; st < steps; ++st)
-        for (int s = 1; s <= 4; ++s) {
-            int rc = enqueue_stage_t<T>(h, c, s);
+        for (int l = 0; l < nl; ++l) {
+            int rc = enqueue_launch_t<T>(h, c, l);
             if (rc != NLSE_OK) return rc;
         }
     return NLSE_OK;
@@ -218,25 +284,34 @@ int enqueue_steps(nlse_solver* h, int steps)
 int step_impl(nlse_solver* h, int steps)
 {
     if (steps < 0) return fail(NLSE_EINVAL, "steps must be >= 0");
-    if (h->gsteps <= 0 || resident1d(h) || steps < h->gsteps) return enqueue_steps(h, steps);
-    if (!h->gexec || h->gcaptured != h->gsteps) {
-        if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-        h->gexec = nullptr;
+    // a fused step swaps U with Y1: replay graphs of an even number of steps,
+    // one per buffer holding U when the replay starts
+    int gs = h->gsteps;
+    if (fusion(h) > 1 && (gs & 1)) gs += 1;
+    if (gs <= 0 || resident1d(h) || steps < gs) return enqueue_steps(h, steps);
+    const int slot = (h->U == h->ubase) ? 0 : 1;
+    if (!h->gexec[slot] || h->gcaptured[slot] != gs) {
+        if (h->gexec[slot]) (void)hipGraphExecDestroy(h->gexec[slot]);
+        h->gexec[slot] = nullptr;
         hipGraph_t g;
+        void* u_before = h->U;
+        void* y1_before = h->Y1;
         HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_steps(h, h->gsteps);
+        int rc = enqueue_steps(h, gs);
         hipError_t e = hipStreamEndCapture(h->stream, &g);
+        h->U = u_before;  // capture records launches without running them
+        h->Y1 = y1_before;
         if (rc != NLSE_OK) return rc;
         if (e != hipSuccess) return fail(NLSE_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+        e = hipGraphInstantiate(&h->gexec[slot], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         if (e != hipSuccess) return fail(NLSE_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        h->gcaptured = h->gsteps;
+        h->gcaptured[slot] = gs;
     }
     int left = steps;
-    while (left >= h->gsteps) {
-        HIPCHK(hipGraphLaunch(h->gexec, h->stream));
-        left -= h->gsteps;
+    while (left >= gs) {
+        HIPCHK(hipGraphLaunch(h->gexec[slot], h->stream));
+        left -= gs;
     }
     return enqueue_steps(h, left);
 }
@@ -369,6 +444,9 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
         nlse_destroy(h);
         return fail(NLSE_EHIP, "stream/event creation failed");
     }
+    h->ubase = h->U;
+    h->fuse = (dim == 2) ? 2 : 1;
+    h->tile = 8;
     set_params(h, BC, s, a, h2, k);
     // a defined initial state: zero field and potential
     (void)hipMemsetAsync(h->U, 0, eb, h->stream);
@@ -382,7 +460,7 @@ void nlse_destroy(nlse_solver* h)
 {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    drop_graphs(h);
     void* bufs[6] = {h->U, h->Y0, h->Y1, h->acc, h->D, h->V};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -448,31 +526,34 @@ int nlse_step_timed(nlse_solver* h, int steps, double* ms)
     return NLSE_OK;
 }
 
-int nlse_profile_stages(nlse_solver* h, int steps, double* ms_stage)
+int nlse_profile_launches(nlse_solver* h, int steps, double* ms_launch, int* nlaunch)
 {
-    if (!h || !ms_stage || steps <= 0) return fail(NLSE_EINVAL, "bad argument");
+    if (!h || !ms_launch || !nlaunch || steps <= 0) return fail(NLSE_EINVAL, "bad argument");
     if (resident1d(h)) return fail(NLSE_EINVAL, "small 1D problems run whole chunks in one kernel");
+    const int nl = 4 / fusion(h);
     hipEvent_t ev[5];
-    for (int i = 0; i < 5; ++i) HIPCHK(hipEventCreate(&ev[i]));
+    for (int i = 0; i <= nl; ++i) HIPCHK(hipEventCreate(&ev[i]));
     double acc[4] = {0, 0, 0, 0};
     int rc = NLSE_OK;
     for (int st = 0; st < steps && rc == NLSE_OK; ++st) {
         if (hipEventRecord(ev[0], h->stream) != hipSuccess) { rc = fail(NLSE_EHIP, "hipEventRecord"); break; }
-        for (int s = 1; s <= 4 && rc == NLSE_OK; ++s) {
-            rc = enqueue_stage(h, s);
-            if (rc == NLSE_OK && hipEventRecord(ev[s], h->stream) != hipSuccess) rc = fail(NLSE_EHIP, "hipEventRecord");
+        for (int l = 0; l < nl && rc == NLSE_OK; ++l) {
+            rc = enqueue_launch(h, l);
+            if (rc == NLSE_OK && hipEventRecord(ev[l + 1], h->stream) != hipSuccess)
+                rc = fail(NLSE_EHIP, "hipEventRecord");
         }
         if (rc != NLSE_OK) break;
-        if (hipEventSynchronize(ev[4]) != hipSuccess) { rc = fail(NLSE_EHIP, "hipEventSynchronize"); break; }
-        for (int s = 0; s < 4; ++s) {
+        if (hipEventSynchronize(ev[nl]) != hipSuccess) { rc = fail(NLSE_EHIP, "hipEventSynchronize"); break; }
+        for (int l = 0; l < nl; ++l) {
             float f = 0.f;
-            (void)hipEventElapsedTime(&f, ev[s], ev[s + 1]);
-            acc[s] += f;
+            (void)hipEventElapsedTime(&f, ev[l], ev[l + 1]);
+            acc[l] += f;
         }
     }
-    for (int i = 0; i < 5; ++i) (void)hipEventDestroy(ev[i]);
+    for (int i = 0; i <= nl; ++i) (void)hipEventDestroy(ev[i]);
     if (rc != NLSE_OK) return rc;
-    for (int s = 0; s < 4; ++s) ms_stage[s] = acc[s] / steps;
+    for (int l = 0; l < nl; ++l) ms_launch[l] = acc[l] / steps;
+    *nlaunch = nl;
     return NLSE_OK;
 }
 
@@ -483,10 +564,102 @@ int nlse_sync(nlse_solver* h)
     return NLSE_OK;
 }
 
+int nlse_set_fusion(nlse_solver* h, int stages_per_launch)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    if (stages_per_launch != 1 && stages_per_launch != 2 && stages_per_launch != 4)
+        return fail(NLSE_EINVAL, "stages per launch must be 1, 2 or 4");
+    if (h->dim != 2 && stages_per_launch != 1) return fail(NLSE_EINVAL, "stage fusion is implemented for 2D");
+    h->fuse = stages_per_launch;
+    drop_graphs(h);
+    return NLSE_OK;
+}
+
+int nlse_set_tile(nlse_solver* h, int tile_rows)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32) return fail(NLSE_EINVAL, "tile rows must be 8, 16 or 32");
+    h->tile = tile_rows;
+    drop_graphs(h);
+    return NLSE_OK;
+}
+
+int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, double* best_ms)
+{
+    if (!h || steps <= 0) return fail(NLSE_EINVAL, "bad argument");
+    if (h->dim != 2) {  // 1D and 3D have one kernel shape
+        if (best_fusion) *best_fusion = 1;
+        if (best_tile) *best_tile = 0;
+        if (best_ms) *best_ms = 0;
+        return NLSE_OK;
+    }
+    const size_t bytes = elem_bytes(h->prec) * (size_t)h->n;
+    void* saved = nullptr;
+    if (hipMalloc(&saved, bytes) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    HIPCHK(hipMemcpyAsync(saved, h->U, bytes, hipMemcpyDeviceToDevice, h->stream));
+    const int fz[3] = {1, 2, 4}, tl[3] = {8, 16, 32};
+    const int gsave = h->gsteps;
+    h->gsteps = 0;
+    int bf = h->fuse, bt = h->tile, rc = NLSE_OK;
+    double bms = 1e30;
+    for (int a = 0; a < 3 && rc == NLSE_OK; ++a)
+        for (int b = 0; b < 3 && rc == NLSE_OK; ++b) {
+            h->fuse = fz[a];
+            h->tile = tl[b];
+            rc = enqueue_steps(h, 1);  // warm-up (first launch of this kernel)
+            double ms = 0;
+            if (rc == NLSE_OK) rc = nlse_step_timed(h, steps, &ms);
+            if (rc == NLSE_OK && ms < bms) { bms = ms; bf = fz[a]; bt = tl[b]; }
+            if (hipMemcpyAsync(h->U, saved, bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess && rc == NLSE_OK)
+                rc = fail(NLSE_EHIP, "restore after tuning trial");
+        }
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(saved);
+    h->gsteps = gsave;
+    h->fuse = bf;
+    h->tile = bt;
+    drop_graphs(h);
+    if (rc != NLSE_OK) return rc;
+    if (best_fusion) *best_fusion = bf;
+    if (best_tile) *best_tile = bt;
+    if (best_ms) *best_ms = bms / steps;
+    return NLSE_OK;
+}
+
 int nlse_set_graph(nlse_solver* h, int steps_per_graph)
 {
     if (!h || steps_per_graph < 0) return fail(NLSE_EINVAL, "bad argument");
     h->gsteps = steps_per_graph;
+    return NLSE_OK;
+}
+
+int nlse_copy_probe(long long bytes, int reps, double* ms)
+{
+    if (bytes <= 0 || bytes % 16 != 0 || reps <= 0 || !ms) return fail(NLSE_EINVAL, "bad argument");
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0, e1;
+    HIPCHK(hipMalloc(&a, (size_t)bytes));
+    if (hipMalloc(&b, (size_t)bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(NLSE_ENOMEM, "hipMalloc failed");
+    }
+    (void)hipMemset(a, 0, (size_t)bytes);
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const long n = bytes / 16;
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(copy_probe_kernel, dim3(8192), dim3(256), 0, 0, (const double2*)a, (double2*)b, n);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float f = 0.f;
+    (void)hipEventElapsedTime(&f, e0, e1);
+    *ms = f / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(NLSE_EHIP, hipGetErrorString(e));
     return NLSE_OK;
 }
 

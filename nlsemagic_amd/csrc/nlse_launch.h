@@ -19,11 +19,18 @@ template <typename T> struct StageArgs {
     Coefs<T> c;
     int N, M, L;  // 1D: N; 2D: N x M; 3D: L x N x M (M contiguous)
     int BC, method, stage;
+    int tile;  // 2D: tile height in rows (8, 16 or 32); 0 = default
 };
 
 template <typename T> hipError_t launch_stage1d(const StageArgs<T>& a, hipStream_t st);
 template <typename T> hipError_t launch_stage2d(const StageArgs<T>& a, hipStream_t st);
 template <typename T> hipError_t launch_stage3d(const StageArgs<T>& a, hipStream_t st);
+
+// 2D temporal fusion: RK4 stages 1-2 (first) or 3-4 of a step in one launch
+// (yin with a halo of 2R; writes Y_2 + ktot, or the new U to yout != u), and
+// a whole step in one launch (yin == u; the new U goes to yout != u).
+template <typename T> hipError_t launch_pair2d(const StageArgs<T>& a, bool first, hipStream_t st);
+template <typename T> hipError_t launch_step2d(const StageArgs<T>& a, hipStream_t st);
 
 // Whole chunk of a small 1D problem in one workgroup (state held on chip).
 template <typename T>

@@ -92,10 +92,12 @@ def test_real_input_gets_zero_imaginary_part(hip, oracle):
     assert bits_equal(got, exp)
 
 
-@pytest.mark.parametrize("graph", [0, 3])
-def test_resident_solver_chunks_equal_one_call(hip, oracle, graph):
+@pytest.mark.parametrize("fusion", [1, 2, 4])
+@pytest.mark.parametrize("graph", [0, 3, 4])
+def test_resident_solver_chunks_equal_one_call(hip, oracle, graph, fusion):
     """State kept on the GPU across chunks = one long call (2D 2SHOC, MSD),
-    with and without hipGraph replay of the chunk loop."""
+    with and without hipGraph replay of the chunk loop, for every fusion
+    depth (odd graph lengths exercise the U/Y1 buffer swap)."""
     rng = np.random.default_rng(8)
     shape = (40, 70)
     U = 1 + 0.1 * rng.uniform(-1, 1, shape) + 0.1j * rng.uniform(-1, 1, shape)
@@ -103,17 +105,56 @@ def test_resident_solver_chunks_equal_one_call(hip, oracle, graph):
     s, a, h2, k = -1.0, 1.0, 0.01, 0.002
     sol = hip.Solver(shape, 2, 2, 2, s, a, h2, k)
     sol.set_graph(graph)
+    sol.set_fusion(fusion)
     sol.set_state(U, V)
-    for _ in range(3):
-        sol.step(7)
+    for n in (7, 1, 9, 4):
+        sol.step(n)
     got = sol.state()
     sol.close()
     exp = oracle.take_steps(U, V, s, a, h2, 2, 21, k, 2)
     assert bits_equal(got, exp)
 
 
+def _fusion_cases():
+    out = []
+    seed = 5000
+    for shape in [(3, 3), (4, 5), (17, 65), (33, 130), (50, 64), (18, 7), (31, 129), (65, 200), (9, 70), (33, 66)]:
+        for method in (1, 2):
+            for prec in (2, 1):
+                for BC in (1, 2, 3, 0):
+                    for fusion in (1, 2, 4):
+                        seed += 1
+                        out.append((shape, method, prec, BC, fusion, (8, 16, 32)[seed % 3], seed))
+    return out
+
+
+@pytest.mark.parametrize("shape,method,prec,BC,fusion,tile,seed", _fusion_cases())
+def test_fused_2d_matches_oracle(hip, oracle, shape, method, prec, BC, fusion, tile, seed):
+    """One, two and four RK4 stages per launch, on 8-, 16- and 32-row tiles,
+    give the reference's bits."""
+    _announce("fused", shape, method, prec, BC, fusion, tile, seed)
+    rng = np.random.default_rng(seed)
+    U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
+    V = 0.5 * rng.uniform(-1, 1, shape)
+    h2 = 0.04
+    k = 0.1 * h2
+    s = -1.0 if BC != 3 else 0.8
+    exp = oracle.take_steps(U, V, s, 0.9, h2, BC, 3, k, method, precision=prec)
+    sol = hip.Solver(shape, method, prec, BC, s, 0.9, h2, k)
+    sol.set_fusion(fusion)
+    sol.set_tile(tile)
+    sol.set_state(U, V)
+    sol.step(3)
+    got = sol.state()
+    sol.close()
+    if prec == 1:  # the resident API widens single-precision state to double on download
+        got = got.astype(np.complex64).astype(np.complex128)
+    assert bits_equal(got, exp), f"max |diff| {np.max(np.abs(got - exp))}"
+
+
+@pytest.mark.parametrize("fusion", [1, 2, 4])
 @pytest.mark.parametrize("method", [1, 2])
-def test_full_size_2d_bitwise_short_run(hip, oracle, method):
+def test_full_size_2d_bitwise_short_run(hip, oracle, method, fusion):
     """1024 x 1024 dark vortex, MSD, a few steps: bit-exact at full size."""
     n = 1024
     h = 0.1
@@ -124,8 +165,13 @@ def test_full_size_2d_bitwise_short_run(hip, oracle, method):
     U = np.tanh(r) * np.exp(1j * np.arctan2(Y, X)) + 1e-3 * (rng.uniform(-1, 1, r.shape) + 1j * rng.uniform(-1, 1, r.shape))
     V = np.zeros(r.shape)
     k = (0.3 if method == 1 else 0.2121) * h * h
-    _announce("full2d", method)
-    got = hip.take_steps(U, V, -1.0, 1.0, h * h, 2, 3, k, precision=2, method=method)
+    _announce("full2d", method, fusion)
+    sol = hip.Solver((n, n), method, 2, 2, -1.0, 1.0, h * h, k)
+    sol.set_fusion(fusion)
+    sol.set_state(U, V)
+    sol.step(3)
+    got = sol.state()
+    sol.close()
     exp = oracle.take_steps(U, V, -1.0, 1.0, h * h, 2, 3, k, method)
     assert bits_equal(got, exp), rel_l2(got, exp)
 
