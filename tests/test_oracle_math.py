@@ -1,0 +1,272 @@
+"""Mathematical pins of the CPU oracle.
+
+The 1D and 2D oracle is pinned bit-for-bit to the compiled reference
+(test_oracle_golden.py).  The reference's 3D integrators cannot be built here
+(they include MATLAB's mex.h), so the 3D oracle is pinned by what the
+mathematics fixes; every check below also runs on 1D and 2D, where passing
+shows the check itself is right:
+
+- eigen-KAT: with Dirichlet boundaries, s = 0 and V = 0, a product of sine
+  modes is an exact eigenvector of the discrete operator; one RK4 step
+  multiplies it by R(z) = 1 + z + z^2/2 + z^3/6 + z^4/24, z = i a k sigma.
+  sigma is the symbol of the CD / 2SHOC stencil, and is itself checked to be
+  a 2nd / 4th order approximation of -|theta|^2/h^2;
+- convergence: the linear harmonic ground state exp(-r^2/2a), V = r^2/a,
+  evolves as exp(-i d t) (NLSEmagic1D_FRS.m:540, NLSE2D.m:1011,
+  NLSE3D.m:1272); the observed order is 2 (CD) and 4 (2SHOC);
+- the dark soliton tanh(x/sqrt 2) e^{-it} (s = -1, MSD; NLSEmagic1D_FRS.m:538);
+- a uniform field with Lap0 boundaries follows u' = i(s|u|^2 - v)u, which
+  fixes the signs of the nonlinear and potential terms;
+- boundary identities of one F evaluation (Dirichlet, Lap0, MSD);
+- 3D CD on an L = 3 grid with zero Dirichlet end planes equals 2D CD with
+  V + 2a/h^2 on the middle plane;
+- one 3D F evaluation equals a restatement written with axis-shift operators.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+
+def R4(z):
+    return 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+
+
+def symbol(method, thetas, h2):
+    c = [np.cos(t) for t in thetas]
+    d = len(thetas)
+    lam = (2 * sum(c) - 2 * d) / h2
+    if method == 1:
+        return lam
+    if d == 1:
+        return lam * (7 - c[0]) / 6
+    if d == 2:
+        return lam * (1 - (c[0] + c[1]) / 6) + (4 * c[0] * c[1] - 4) / (6 * h2)
+    pairs = c[0] * c[1] + c[1] * c[2] + c[2] * c[0]
+    return -lam * (2 * sum(c) - 10) / 12 + (4 * pairs - 12) / (6 * h2)
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("method,order", [(1, 2), (2, 4)])
+def test_symbol_is_consistent_with_the_laplacian(d, method, order):
+    """sigma(theta) h^2 = -|theta|^2 + O(theta^(order+2)), any direction."""
+    rng = np.random.default_rng(d)
+    direction = rng.uniform(0.3, 1.0, d)
+    errs = []
+    for eps in (0.08, 0.04):
+        th = eps * direction
+        errs.append(abs(symbol(method, th, 1.0) + np.sum(th ** 2)))
+    observed = np.log2(errs[0] / errs[1])
+    assert abs(observed - (order + 2)) < 0.1, observed
+
+
+def test_2shoc_symbol_at_nyquist_is_four_thirds_of_cd():
+    """The 3/4 factor of the reference's 2SHOC time-step rule (NLSEmagic2D.m:131-134)."""
+    for d in (1, 2, 3):
+        th = [np.pi] * d
+        assert np.isclose(symbol(2, th, 1.0), 4 / 3 * symbol(1, th, 1.0))
+
+
+def _shape(d):
+    return {1: (23,), 2: (13, 17), 3: (9, 11, 14)}[d]  # (N,), (N, M), (L, N, M)
+
+
+def _mode(shape, modes):
+    axes = [np.sin(np.pi * m * np.arange(n) / (n - 1)) for n, m in zip(shape, modes)]
+    out = axes[0]
+    for a in axes[1:]:
+        out = np.multiply.outer(out, a)
+    return out
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("method", [1, 2])
+def test_eigenmode_known_answer(oracle, d, method):
+    shape = _shape(d)
+    modes = {1: (3,), 2: (2, 5), 3: (1, 3, 4)}[d]
+    thetas = [np.pi * m / (n - 1) for n, m in zip(shape, modes)]
+    a, h2, k, steps = 0.7, 0.04, 0.005, 9
+    sig = symbol(method, thetas, h2)
+    U0 = (1 + 0.5j) * _mode(shape, modes)
+    V = np.zeros(shape)
+    got = oracle.take_steps(U0, V, 0.0, a, h2, 1, steps, k, method)
+    exp = R4(1j * a * k * sig) ** steps * U0
+    assert rel_l2(got, exp) < 1e-12
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("method", [1, 2])
+def test_eigenmode_single_precision(oracle, d, method):
+    shape = _shape(d)
+    modes = {1: (3,), 2: (2, 5), 3: (1, 3, 4)}[d]
+    thetas = [np.pi * m / (n - 1) for n, m in zip(shape, modes)]
+    a, h2, k, steps = 0.7, 0.04, 0.005, 9
+    U0 = (1 + 0.5j) * _mode(shape, modes)
+    got = oracle.take_steps(U0, np.zeros(shape), 0.0, a, h2, 1, steps, k, method, precision=1)
+    exp = R4(1j * a * k * symbol(method, thetas, h2)) ** steps * U0
+    assert rel_l2(got, exp) < 1e-5
+
+
+def _harmonic_error(oracle, d, method, n, a=0.8, L=7.0, T=0.25):
+    h = 2 * L / (n - 1)
+    x = -L + h * np.arange(n)
+    grids = np.meshgrid(*([x] * d), indexing="ij")
+    r2 = sum(g * g for g in grids)
+    U0 = np.exp(-r2 / (2 * a)) + 0j
+    V = r2 / a
+    k = 0.05 * h * h / a
+    steps = int(round(T / k))
+    k = T / steps
+    got = oracle.take_steps(U0, V, 0.0, a, h * h, 1, steps, k, method)
+    exact = U0 * np.exp(-1j * d * T)
+    return np.max(np.abs(got - exact))
+
+
+@pytest.mark.parametrize("d,sizes", [(1, (61, 121)), (2, (41, 81)), (3, (25, 49))])
+@pytest.mark.parametrize("method,order", [(1, 2), (2, 4)])
+def test_harmonic_ground_state_convergence_order(oracle, d, sizes, method, order):
+    e1 = _harmonic_error(oracle, d, method, sizes[0])
+    e2 = _harmonic_error(oracle, d, method, sizes[1])
+    h_ratio = (sizes[1] - 1) / (sizes[0] - 1)
+    observed = np.log(e1 / e2) / np.log(h_ratio)
+    assert abs(observed - order) < 0.35, (e1, e2, observed)
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_dark_soliton_1d(oracle, method):
+    n = 801
+    L = 25.0
+    h = 2 * L / (n - 1)
+    x = -L + h * np.arange(n)
+    U0 = np.tanh(x / np.sqrt(2)) + 0j
+    k = 0.2 * h * h
+    T = 1.0
+    steps = int(round(T / k))
+    k = T / steps
+    got = oracle.take_steps(U0, np.zeros(n), -1.0, 1.0, h * h, 2, steps, k, method)
+    err = np.max(np.abs(got - U0 * np.exp(-1j * T)))
+    assert err < (2e-4 if method == 1 else 2e-6), err
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("precision", [2, 1])
+def test_uniform_field_follows_the_local_ode(oracle, d, method, precision):
+    shape = _shape(d)
+    c0 = 0.6 - 0.8j
+    s, v0, k, steps = -1.3, 0.4, 0.01, 50
+    U0 = np.full(shape, c0)
+    V = np.full(shape, v0)
+    got = oracle.take_steps(U0, V, s, 0.9, 0.05, 3, steps, k, method, precision=precision)
+    exact = c0 * np.exp(1j * (s * abs(c0) ** 2 - v0) * k * steps)
+    # the discrete Laplacian of a constant is zero up to rounding of sums like c - 4c
+    assert np.allclose(got, got.ravel()[0], rtol=0, atol=1e-12 if precision == 2 else 1e-5)
+    # the classical RK4 tableau on the scalar ODE, written as a generic method
+    f = lambda u: 1j * (s * abs(u) ** 2 - v0) * u
+    rk = c0
+    for _ in range(steps):
+        k1 = f(rk)
+        k2 = f(rk + k / 2 * k1)
+        k3 = f(rk + k / 2 * k2)
+        k4 = f(rk + k * k3)
+        rk = rk + k / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+    assert abs(got.ravel()[0] - rk) < (1e-12 if precision == 2 else 2e-6)
+    assert abs(got.ravel()[0] - exact) < (1e-8 if precision == 2 else 1e-6)
+    wrong = c0 * np.exp(1j * (-s * abs(c0) ** 2 + v0) * k * steps)
+    assert abs(got.ravel()[0] - wrong) > 0.1
+
+
+def _boundary_points(shape):
+    for idx in itertools.product(*[range(n) for n in shape]):
+        if any(i == 0 or i == n - 1 for i, n in zip(idx, shape)):
+            yield idx
+
+
+@pytest.mark.parametrize("d", [1, 2, 3])
+@pytest.mark.parametrize("method", [1, 2])
+def test_boundary_identities_of_one_evaluation(oracle, d, method):
+    shape = _shape(d)
+    rng = np.random.default_rng(d * 10 + method)
+    U = 1 + 0.3 * rng.uniform(-1, 1, shape) + 0.3j * rng.uniform(-1, 1, shape)
+    V = rng.uniform(-1, 1, shape)
+    s, a, h2 = -0.7, 0.9, 0.05
+    F1 = oracle.rhs(U, V, s, a, h2, 1, method)
+    F2 = oracle.rhs(U, V, s, a, h2, 2, method)
+    F3 = oracle.rhs(U, V, s, a, h2, 3, method)
+    for b in _boundary_points(shape):
+        q = tuple(min(max(i, 1), n - 2) for i, n in zip(b, shape))
+        assert F1[b] == 0
+        assert np.isclose(F3[b], 1j * (s * abs(U[b]) ** 2 - V[b]) * U[b], rtol=1e-12, atol=0)
+        om = (F2[q] / U[q]).imag  # MSD: the partner's phase rate carried to the boundary
+        assert np.isclose(F2[b], 1j * om * U[b], rtol=1e-13, atol=1e-15)
+
+
+def test_3d_cd_reduces_to_2d_with_shifted_potential(oracle):
+    rng = np.random.default_rng(11)
+    N, M = 12, 15
+    a, h2, s, k = 0.8, 0.05, -1.0, 0.004
+    U2 = 1 + 0.3 * rng.uniform(-1, 1, (N, M)) + 0.3j * rng.uniform(-1, 1, (N, M))
+    V2 = rng.uniform(-1, 1, (N, M))
+    U3 = np.zeros((3, N, M), complex)
+    U3[1] = U2
+    V3 = np.zeros((3, N, M))
+    V3[1] = V2
+    got = oracle.take_steps(U3, V3, s, a, h2, 1, 6, k, 1)
+    exp = oracle.take_steps(U2, V2 + 2 * a / h2, s, a, h2, 1, 6, k, 1)
+    assert np.all(got[0] == 0) and np.all(got[2] == 0)
+    assert rel_l2(got[1], exp) < 1e-13
+
+
+def _shift(U, axis, off):
+    out = np.zeros_like(U)
+    n = U.shape[axis]
+    src = [slice(None)] * U.ndim
+    dst = [slice(None)] * U.ndim
+    if off > 0:
+        src[axis], dst[axis] = slice(off, n), slice(0, n - off)
+    else:
+        src[axis], dst[axis] = slice(0, n + off), slice(-off, n)
+    out[tuple(dst)] = U[tuple(src)]
+    return out
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_3d_rhs_equals_shift_operator_form(oracle, method):
+    """F = i[a L U + (s|U|^2 - V) U] inside, 0 on a Dirichlet boundary, with
+    L = Lap_CD (CD) or L = D/3 - (h^2/12) Lap_CD(D) + (sum_{a<b} T_a T_b - 12)/(6h^2)
+    (2SHOC), T_a = shift(+1) + shift(-1) along axis a and D = Lap_CD(U) inside,
+    D = (V - s|U|^2) U / a on the boundary."""
+    shape = (7, 9, 10)
+    rng = np.random.default_rng(4)
+    U = 1 + 0.3 * rng.uniform(-1, 1, shape) + 0.3j * rng.uniform(-1, 1, shape)
+    V = rng.uniform(-1, 1, shape)
+    s, a, h2 = -0.6, 0.85, 0.05
+    inner = np.zeros(shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    T = [lambda X, ax=ax: _shift(X, ax, 1) + _shift(X, ax, -1) for ax in range(3)]
+    lap = lambda X: (T[0](X) + T[1](X) + T[2](X) - 6 * X) / h2
+    nl = (s * np.abs(U) ** 2 - V) * U
+    if method == 1:
+        Lu = lap(U)
+    else:
+        D = np.where(inner, lap(U), (V - s * np.abs(U) ** 2) * U / a)
+        edges = T[0](T[1](U)) + T[1](T[2](U)) + T[2](T[0](U))
+        Lu = D / 3 - h2 / 12 * lap(D) + (edges - 12 * U) / (6 * h2)
+    exp = np.where(inner, 1j * (a * Lu + nl), 0)
+    got = oracle.rhs(U, V, s, a, h2, 1, method)
+    assert np.allclose(got, exp, rtol=1e-11, atol=1e-11 * np.max(np.abs(exp)))
+
+
+def test_single_precision_tracks_double(oracle):
+    rng = np.random.default_rng(2)
+    for shape in ((64,), (20, 24), (8, 10, 12)):
+        U = 1 + 0.1 * rng.uniform(-1, 1, shape) + 0.1j * rng.uniform(-1, 1, shape)
+        V = 0.1 * rng.uniform(-1, 1, shape)
+        for method in (1, 2):
+            d = oracle.take_steps(U, V, -1.0, 1.0, 0.04, 2, 10, 0.004, method)
+            f = oracle.take_steps(U, V, -1.0, 1.0, 0.04, 2, 10, 0.004, method, precision=1)
+            assert rel_l2(f, d) < 1e-5
