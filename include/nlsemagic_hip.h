@@ -85,7 +85,9 @@ int NLSE3D_TAKE_STEPS_HIP_F(const float *Uoldr, const float *Uoldi, const float 
 typedef struct nlse_solver nlse_solver;
 
 /* precision: 2 = double, 1 = single (the reference's `precision` codes,
- * src/NLSEmagic1D/NLSEmagic1D.m:22-24).  dims: 1D (N), 2D (N, M), 3D (N, M, L)
+ * src/NLSEmagic1D/NLSEmagic1D.m:22-24), 3 = single-precision stages with the
+ * state kept as a compensated hi + lo pair of floats (final RK4 combine by
+ * TwoSum; no counterpart in the reference).  dims: 1D (N), 2D (N, M), 3D (N, M, L)
  * with unused sizes passed as 1. */
 int nlse_create(nlse_solver **out, int dim, int N, int M, int L, int method, int precision,
                 int BC, double s, double a, double h2, double k);
@@ -122,6 +124,33 @@ int nlse_autotune(nlse_solver *h, int steps, int *best_fusion, int *best_tile, d
 int nlse_set_graph(nlse_solver *h, int steps_per_graph);
 /* device bytes held by a solver of this shape (admission control) */
 long long nlse_footprint_bytes(int dim, int N, int M, int L, int method, int precision);
+
+/* ---- batched 2D API ----
+ * nprob independent 2D problems (sizes N[p] x M[p], each with its own s, a,
+ * h^2, k and potential V, one boundary code for all) advanced together: one
+ * persistent kernel launch per fused group of RK4 stages walks the tiles of
+ * every problem.  precision: 2 = double (bit-identical to the reference CPU
+ * integrator, problem by problem), 3 = single with a compensated hi + lo
+ * state.  Problems are addressed by index p; states are double planes. */
+typedef struct nlse_batch nlse_batch;
+int nlse_batch_create(nlse_batch **out, int nprob, const int *N, const int *M, const double *s,
+                      const double *a, const double *h2, const double *k, int method,
+                      int precision, int BC);
+void nlse_batch_destroy(nlse_batch *b);
+long long nlse_batch_points(const nlse_batch *b);
+int nlse_batch_set_state(nlse_batch *b, int p, const double *Ur, const double *Ui, const double *V);
+int nlse_batch_get_state(nlse_batch *b, int p, double *Ur, double *Ui);
+int nlse_batch_step(nlse_batch *b, int steps);
+int nlse_batch_step_timed(nlse_batch *b, int steps, double *ms);
+int nlse_batch_sync(nlse_batch *b);
+/* stages per launch (1, 2, 4), tile rows (8, 16, 32), persistent grid
+ * (multiple of 8 workgroups) and hipGraph steps per replay (0 = off) */
+int nlse_batch_configure(nlse_batch *b, int stages_per_launch, int tile_rows, int grid,
+                         int steps_per_graph);
+/* time `steps` steps for each (stages per launch, tile rows, grid) on a saved
+ * copy of the state, keep the fastest, restore the state */
+int nlse_batch_autotune(nlse_batch *b, int steps, int *best_fusion, int *best_tile, int *best_grid,
+                        double *best_ms);
 
 /* measurement helper: mean device ms of a plain streaming copy of `bytes`
  * (multiple of 16) between two fresh device buffers, over `reps` launches;

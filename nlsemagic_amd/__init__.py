@@ -30,7 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnlsemagic_hip.so")
 
 __all__ = [
-    "lib", "Solver", "HipError",
+    "lib", "Solver", "Batch", "HipError",
     "NLSEmagic1D_Take_Steps_GPU", "NLSEmagic2D_Take_Steps_GPU", "NLSEmagic3D_Take_Steps_GPU",
 ]
 
@@ -90,6 +90,28 @@ def lib():
     L.nlse_sync.argtypes = [C.c_void_p]
     L.nlse_set_graph.restype = C.c_int
     L.nlse_set_graph.argtypes = [C.c_void_p, C.c_int]
+    _I = C.POINTER(C.c_int)
+    L.nlse_batch_create.restype = C.c_int
+    L.nlse_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, _I, _I, _D, _D, _D, _D, C.c_int, C.c_int,
+                                    C.c_int]
+    L.nlse_batch_destroy.restype = None
+    L.nlse_batch_destroy.argtypes = [C.c_void_p]
+    L.nlse_batch_points.restype = C.c_longlong
+    L.nlse_batch_points.argtypes = [C.c_void_p]
+    L.nlse_batch_set_state.restype = C.c_int
+    L.nlse_batch_set_state.argtypes = [C.c_void_p, C.c_int, _D, _D, _D]
+    L.nlse_batch_get_state.restype = C.c_int
+    L.nlse_batch_get_state.argtypes = [C.c_void_p, C.c_int, _D, _D]
+    L.nlse_batch_step.restype = C.c_int
+    L.nlse_batch_step.argtypes = [C.c_void_p, C.c_int]
+    L.nlse_batch_step_timed.restype = C.c_int
+    L.nlse_batch_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.nlse_batch_sync.restype = C.c_int
+    L.nlse_batch_sync.argtypes = [C.c_void_p]
+    L.nlse_batch_configure.restype = C.c_int
+    L.nlse_batch_configure.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.nlse_batch_autotune.restype = C.c_int
+    L.nlse_batch_autotune.argtypes = [C.c_void_p, C.c_int, _I, _I, _I, C.POINTER(C.c_double)]
     L.nlse_copy_probe.restype = C.c_int
     L.nlse_copy_probe.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.nlse_footprint_bytes.restype = C.c_longlong
@@ -294,6 +316,82 @@ class Solver:
 
     def set_graph(self, steps_per_graph):
         _check(self._lib.nlse_set_graph(self._h, int(steps_per_graph)))
+
+
+class Batch:
+    """Many independent 2D problems advanced together (the nlse_batch_* ABI).
+
+    shapes: list of (N, M); s, a, h2, k: per-problem sequences (or scalars).
+    precision 2 (double) or 3 (single, compensated).
+    """
+
+    def __init__(self, shapes, s, a, h2, k, method=2, precision=3, BC=1):
+        n = len(shapes)
+        per = lambda x: np.broadcast_to(np.asarray(x, dtype=np.float64), (n,)).copy()
+        self.shapes = [tuple(map(int, sh)) for sh in shapes]
+        Ns = np.array([sh[0] for sh in self.shapes], dtype=np.int32)
+        Ms = np.array([sh[1] for sh in self.shapes], dtype=np.int32)
+        self._arrays = [per(s), per(a), per(h2), per(k)]
+        self._h = C.c_void_p()
+        self._lib = lib()
+        _check(self._lib.nlse_batch_create(C.byref(self._h), n, _p(Ns, C.c_int), _p(Ms, C.c_int),
+                                           *[_p(x, C.c_double) for x in self._arrays], int(method),
+                                           int(precision), int(BC)))
+
+    def close(self):
+        if self._h:
+            self._lib.nlse_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def points(self):
+        return int(self._lib.nlse_batch_points(self._h))
+
+    def set_state(self, p, U, V=None):
+        U = np.asarray(U)
+        if U.shape != self.shapes[p]:
+            raise ValueError("shape mismatch")
+        ur = np.ascontiguousarray(U.real, dtype=np.float64)
+        ui = np.ascontiguousarray(U.imag if np.iscomplexobj(U) else np.zeros(U.shape), dtype=np.float64)
+        vp = None
+        if V is not None:
+            v = np.ascontiguousarray(V, dtype=np.float64)
+            vp = _p(v, C.c_double)
+        _check(self._lib.nlse_batch_set_state(self._h, int(p), _p(ur, C.c_double), _p(ui, C.c_double), vp))
+
+    def state(self, p):
+        outr = np.empty(self.shapes[p])
+        outi = np.empty(self.shapes[p])
+        _check(self._lib.nlse_batch_get_state(self._h, int(p), _p(outr, C.c_double), _p(outi, C.c_double)))
+        return _cplx(outr, outi)
+
+    def step(self, steps):
+        _check(self._lib.nlse_batch_step(self._h, int(steps)))
+
+    def step_timed(self, steps):
+        ms = C.c_double()
+        _check(self._lib.nlse_batch_step_timed(self._h, int(steps), C.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        _check(self._lib.nlse_batch_sync(self._h))
+
+    def configure(self, stages_per_launch=2, tile_rows=8, grid=1024, steps_per_graph=0):
+        _check(self._lib.nlse_batch_configure(self._h, int(stages_per_launch), int(tile_rows), int(grid),
+                                              int(steps_per_graph)))
+
+    def autotune(self, steps=3):
+        """Pick (stages per launch, tile rows, grid); returns them and ms/step."""
+        f, t, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        _check(self._lib.nlse_batch_autotune(self._h, int(steps), C.byref(f), C.byref(t), C.byref(g),
+                                             C.byref(ms)))
+        return f.value, t.value, g.value, ms.value
 
 
 def copy_probe(nbytes, reps=10):

@@ -105,7 +105,7 @@ template <typename T, int METHOD>
 __global__ __launch_bounds__(256) void stage1d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
                                                       c2<T>* __restrict__ acc, c2<T>* yout,
                                                       const T* __restrict__ V, Coefs<T> c, int N,
-                                                      int toff, int BC, int stage)
+                                                      int toff, int BC, int stage, c2<T>* ulo)
 {
     constexpr int TN = 256, H = 3;
     __shared__ c2<T> sY[TN + 2 * H];
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void stage1d_kernel(const c2<T>* __restrict__ 
     if (gi < 0 || gi >= N) return;
     const c2<T> f = rhs.f_at(gi);
     const c2<T> uu = (stage == 1) ? Y(gi) : u[gi];
-    rk4_epilogue<T>(stage, c, f, uu, acc, yout, yout, gi);
+    rk4_epilogue<T>(stage, c, f, uu, acc, yout, yout, gi, ulo);
 }
 
 template <typename T>
@@ -142,10 +142,10 @@ hipError_t launch_stage1d(const StageArgs<T>& a, hipStream_t st)
     const int nb = (a.N + toff + 255) / 256;
     if (a.method == 2)
         hipLaunchKernelGGL((stage1d_kernel<T, 2>), dim3(nb), dim3(256), 0, st, a.yin, a.u, a.acc, a.yout, a.V,
-                           a.c, a.N, toff, a.BC, a.stage);
+                           a.c, a.N, toff, a.BC, a.stage, a.ulo);
     else
         hipLaunchKernelGGL((stage1d_kernel<T, 1>), dim3(nb), dim3(256), 0, st, a.yin, a.u, a.acc, a.yout, a.V,
-                           a.c, a.N, toff, a.BC, a.stage);
+                           a.c, a.N, toff, a.BC, a.stage, a.ulo);
     return hipGetLastError();
 }
 
@@ -157,18 +157,21 @@ constexpr int RES_PPT = 16;  // points per thread -> N <= 4096
 int chunk1d_resident_max_points(int elem_bytes) { (void)elem_bytes; return RES_THREADS * RES_PPT; }
 
 template <typename T, int METHOD, int PPT>
-__global__ __launch_bounds__(RES_THREADS) void chunk1d_resident_kernel(c2<T>* u_io, const T* __restrict__ Vg,
-                                                                       Coefs<T> c, int N, int BC, int steps)
+__global__ __launch_bounds__(RES_THREADS) void chunk1d_resident_kernel(c2<T>* u_io, c2<T>* ulo_io,
+                                                                       const T* __restrict__ Vg, Coefs<T> c, int N,
+                                                                       int BC, int steps)
 {
     constexpr int NMAX = RES_THREADS * PPT;
     __shared__ c2<T> sY[NMAX];
     __shared__ c2<T> sD[(METHOD == 2) ? NMAX : 1];
     const int tid = threadIdx.x;
-    c2<T> U[PPT], A[PPT], Y[PPT];
+    c2<T> U[PPT], A[PPT], Y[PPT], Lo[PPT];
+    const bool comp = ulo_io != nullptr;
 #pragma unroll
     for (int r = 0; r < PPT; ++r) {
         const int i = tid + RES_THREADS * r;
         U[r] = (i < N) ? u_io[i] : mk<T>(0, 0);
+        Lo[r] = (comp && i < N) ? ulo_io[i] : mk<T>(0, 0);
         Y[r] = U[r];
         A[r] = mk<T>(0, 0);
     }
@@ -204,7 +207,12 @@ __global__ __launch_bounds__(RES_THREADS) void chunk1d_resident_kernel(c2<T>* u_
                 } else if (stage == 4) {
                     A[r].x = A[r].x + (T)1.0 * f.x;
                     A[r].y = A[r].y + (T)1.0 * f.y;
-                    U[r] = mk<T>(U[r].x + c.k6 * A[r].x, U[r].y + c.k6 * A[r].y);
+                    if (comp) {
+                        comp_add<T>(U[r].x, Lo[r].x, c.k6 * A[r].x);
+                        comp_add<T>(U[r].y, Lo[r].y, c.k6 * A[r].y);
+                    } else {
+                        U[r] = mk<T>(U[r].x + c.k6 * A[r].x, U[r].y + c.k6 * A[r].y);
+                    }
                     Y[r] = U[r];
                 } else {
                     A[r].x = A[r].x + (T)2.0 * f.x;
@@ -220,37 +228,38 @@ __global__ __launch_bounds__(RES_THREADS) void chunk1d_resident_kernel(c2<T>* u_
     for (int r = 0; r < PPT; ++r) {
         const int i = tid + RES_THREADS * r;
         if (i < N) u_io[i] = U[r];
+        if (comp && i < N) ulo_io[i] = Lo[r];
     }
 }
 
 template <typename T, int PPT>
-static hipError_t launch_res(c2<T>* u, const T* V, Coefs<T> c, int N, int BC, int method, int steps,
+static hipError_t launch_res(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c, int N, int BC, int method, int steps,
                              hipStream_t st)
 {
     if (method == 2)
-        hipLaunchKernelGGL((chunk1d_resident_kernel<T, 2, PPT>), dim3(1), dim3(RES_THREADS), 0, st, u, V, c, N,
-                           BC, steps);
+        hipLaunchKernelGGL((chunk1d_resident_kernel<T, 2, PPT>), dim3(1), dim3(RES_THREADS), 0, st, u, ulo, V, c,
+                           N, BC, steps);
     else
-        hipLaunchKernelGGL((chunk1d_resident_kernel<T, 1, PPT>), dim3(1), dim3(RES_THREADS), 0, st, u, V, c, N,
-                           BC, steps);
+        hipLaunchKernelGGL((chunk1d_resident_kernel<T, 1, PPT>), dim3(1), dim3(RES_THREADS), 0, st, u, ulo, V, c,
+                           N, BC, steps);
     return hipGetLastError();
 }
 
 template <typename T>
-hipError_t launch_chunk1d_resident(c2<T>* u, const T* V, Coefs<T> c, int N, int BC, int method, int steps,
-                                   hipStream_t st)
+hipError_t launch_chunk1d_resident(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c, int N, int BC, int method,
+                                   int steps, hipStream_t st)
 {
-    if (N <= RES_THREADS * 2) return launch_res<T, 2>(u, V, c, N, BC, method, steps, st);
-    if (N <= RES_THREADS * 4) return launch_res<T, 4>(u, V, c, N, BC, method, steps, st);
-    if (N <= RES_THREADS * 8) return launch_res<T, 8>(u, V, c, N, BC, method, steps, st);
-    return launch_res<T, RES_PPT>(u, V, c, N, BC, method, steps, st);
+    if (N <= RES_THREADS * 2) return launch_res<T, 2>(u, ulo, V, c, N, BC, method, steps, st);
+    if (N <= RES_THREADS * 4) return launch_res<T, 4>(u, ulo, V, c, N, BC, method, steps, st);
+    if (N <= RES_THREADS * 8) return launch_res<T, 8>(u, ulo, V, c, N, BC, method, steps, st);
+    return launch_res<T, RES_PPT>(u, ulo, V, c, N, BC, method, steps, st);
 }
 
 template hipError_t launch_stage1d<double>(const StageArgs<double>&, hipStream_t);
 template hipError_t launch_stage1d<float>(const StageArgs<float>&, hipStream_t);
-template hipError_t launch_chunk1d_resident<double>(double2*, const double*, Coefs<double>, int, int, int, int,
-                                                    hipStream_t);
-template hipError_t launch_chunk1d_resident<float>(float2*, const float*, Coefs<float>, int, int, int, int,
-                                                   hipStream_t);
+template hipError_t launch_chunk1d_resident<double>(double2*, double2*, const double*, Coefs<double>, int, int,
+                                                    int, int, hipStream_t);
+template hipError_t launch_chunk1d_resident<float>(float2*, float2*, const float*, Coefs<float>, int, int, int,
+                                                   int, hipStream_t);
 
 }  // namespace nlse

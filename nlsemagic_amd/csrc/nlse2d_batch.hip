@@ -1,0 +1,78 @@
+// nlse2d_batch.hip -- many independent 2D problems (uneven sizes, own s, a,
+// h, k and V each) advanced by ONE persistent launch per fused group of RK4
+// stages.  Each workgroup walks the concatenated tile list of all problems;
+// the tile code is the single-problem one (nlse2d_tile.h), so every problem's
+// result is bit-identical to running it alone.
+#include "nlse2d_tile.h"
+
+namespace nlse {
+
+// A persistent grid walks the concatenated tile list of all problems: block b
+// works on XCD b mod 8, which owns a contiguous eighth of the list, so the
+// tiles of one problem (and their shared halos) stay in one L2.
+template <typename T, int METHOD, int TI, int FIRST, int LAST>
+__global__ __launch_bounds__(256) void batch2d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                      c2<T>* __restrict__ acc, c2<T>* out,
+                                                      const T* __restrict__ V, const BatchProb2<T>* __restrict__ probs,
+                                                      const int* __restrict__ tile_prob, int ntiles,
+                                                      c2<T>* __restrict__ ulo)
+{
+    using L = TileLds<T, METHOD, TI, FIRST, LAST>;
+    __shared__ c2<T> sY[L::NY];
+    __shared__ c2<T> sD[L::ND];
+    const int xcd = blockIdx.x & 7, nper = gridDim.x >> 3;
+    const int per = (ntiles + 7) >> 3;
+    for (int i = blockIdx.x >> 3; i < per; i += nper) {
+        const int t = xcd * per + i;
+        if (t >= ntiles) break;
+        const BatchProb2<T>& P = probs[tile_prob[t]];
+        const int lt = t - P.tile0;
+        const int ti = lt / P.tiles_j, tj = lt - ti * P.tiles_j;
+        fused2d_tile<T, METHOD, TI, FIRST, LAST>(sY, sD, ti * TI - P.ioff, tj * TJ2 - P.joff, P.N, P.M,
+                                                 yin + P.off, u + P.off, acc + P.off, out + P.off, V + P.off, P.c,
+                                                 P.BC, ulo ? ulo + P.off : nullptr);
+    }
+}
+
+template <typename T, int METHOD, int TI, int FIRST, int LAST>
+static hipError_t launch_b(const BatchArgs<T>& a, hipStream_t st)
+{
+    hipLaunchKernelGGL((batch2d_kernel<T, METHOD, TI, FIRST, LAST>), dim3(a.grid), dim3(256), 0, st, a.yin, a.u,
+                       a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.ulo);
+    return hipGetLastError();
+}
+
+template <typename T, int METHOD, int FIRST, int LAST>
+static hipError_t launch_b_t(const BatchArgs<T>& a, hipStream_t st)
+{
+    switch (a.tile) {
+    case 8: return launch_b<T, METHOD, 8, FIRST, LAST>(a, st);
+    case 32: return launch_b<T, METHOD, 32, FIRST, LAST>(a, st);
+    default: return launch_b<T, METHOD, 16, FIRST, LAST>(a, st);
+    }
+}
+
+template <typename T, int METHOD>
+static hipError_t launch_b_m(const BatchArgs<T>& a, hipStream_t st)
+{
+    if (a.first == 1 && a.last == 4) return launch_b_t<T, METHOD, 1, 4>(a, st);
+    if (a.first == 1 && a.last == 2) return launch_b_t<T, METHOD, 1, 2>(a, st);
+    if (a.first == 3 && a.last == 4) return launch_b_t<T, METHOD, 3, 4>(a, st);
+    switch (a.first) {
+    case 1: return launch_b_t<T, METHOD, 1, 1>(a, st);
+    case 2: return launch_b_t<T, METHOD, 2, 2>(a, st);
+    case 3: return launch_b_t<T, METHOD, 3, 3>(a, st);
+    default: return launch_b_t<T, METHOD, 4, 4>(a, st);
+    }
+}
+
+template <typename T>
+hipError_t launch_batch2d(const BatchArgs<T>& a, hipStream_t st)
+{
+    return a.method == 2 ? launch_b_m<T, 2>(a, st) : launch_b_m<T, 1>(a, st);
+}
+
+template hipError_t launch_batch2d<double>(const BatchArgs<double>&, hipStream_t);
+template hipError_t launch_batch2d<float>(const BatchArgs<float>&, hipStream_t);
+
+}  // namespace nlse

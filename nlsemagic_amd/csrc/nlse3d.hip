@@ -101,7 +101,7 @@ __global__ __launch_bounds__(256) void d3_kernel(const c2<T>* __restrict__ y, c2
 template <typename T, int METHOD>
 __global__ __launch_bounds__(256) void f3_kernel(const c2<T>* __restrict__ y, const c2<T>* u, c2<T>* __restrict__ acc,
                                                  c2<T>* yout, const c2<T>* __restrict__ D, const T* __restrict__ V,
-                                                 Coefs<T> c, Grid3 g, int BC, int stage)
+                                                 Coefs<T> c, Grid3 g, int BC, int stage, c2<T>* ulo)
 {
     const int k = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void f3_kernel(const c2<T>* __restrict__ y, co
         f = mk<T>(0, 0);
     }
     const c2<T> uu = (stage == 1) ? y[p] : u[p];
-    rk4_epilogue<T>(stage, c, f, uu, acc, yout, yout, p);
+    rk4_epilogue<T>(stage, c, f, uu, acc, yout, yout, p, ulo);
 }
 
 template <typename T>
@@ -136,10 +136,10 @@ hipError_t launch_stage3d(const StageArgs<T>& a, hipStream_t st)
     if (a.method == 2) {
         hipLaunchKernelGGL((d3_kernel<T>), grid, dim3(256), 0, st, a.yin, a.dbuf, a.V, a.c, g, a.BC);
         hipLaunchKernelGGL((f3_kernel<T, 2>), grid, dim3(256), 0, st, a.yin, a.u, a.acc, a.yout, a.dbuf, a.V,
-                           a.c, g, a.BC, a.stage);
+                           a.c, g, a.BC, a.stage, a.ulo);
     } else {
         hipLaunchKernelGGL((f3_kernel<T, 1>), grid, dim3(256), 0, st, a.yin, a.u, a.acc, a.yout,
-                           (const c2<T>*)nullptr, a.V, a.c, g, a.BC, a.stage);
+                           (const c2<T>*)nullptr, a.V, a.c, g, a.BC, a.stage, a.ulo);
     }
     return hipGetLastError();
 }

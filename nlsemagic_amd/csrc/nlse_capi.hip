@@ -9,101 +9,33 @@
 #include <string>
 #include <vector>
 
-#include "../../include/nlsemagic_hip.h"
-#include "nlse_launch.h"
+#include "nlse_host.h"
 
 using namespace nlse;
 
 namespace {
-
 thread_local std::string g_err;
+}
 
-int fail(int code, const std::string& msg)
+namespace nlse {
+int set_error(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
 }
 
-#define HIPCHK(x)                                                                          \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess) return fail(NLSE_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
-    } while (0)
-
-// ---- scalar precompute, in the precision and rounding of each reference file
-
-// NLSEmagic1D_CPU.c:384-390 (double) and :460-466 (float, all float arithmetic)
-template <typename T> void coefs1d(Coefs<T>& c, T s, T a, T h2, T k)
+void split_hilo(const double* x, float* hi, float* lo, long n)
 {
-    c = Coefs<T>{};
-    c.s = s; c.a = a; c.k = k;
-    c.ah2 = a / h2;
-    c.k2 = k / (T)2.0;
-    c.k6 = k / (T)6.0;
-    c.l_a = (T)1.0 / a;
-    c.lh2 = (T)1.0 / h2;
-    c.a76 = a * ((T)7.0 / (T)6.0);
-    c.a112 = a * ((T)1.0 / (T)12.0);
-}
-
-// NLSEmagic2D_CPU.c:757-763 (double) and :865-871 (float: double expressions
-// rounded to float, except ah2 = a/h2, a float division)
-void coefs2d(Coefs<double>& c, double s, double a, double h2, double k)
-{
-    c = Coefs<double>{};
-    c.s = s; c.a = a; c.k = k;
-    c.l_a = 1.0 / a;
-    c.lh2 = 1.0 / h2;
-    c.k2 = k / 2.0;
-    c.k6 = k / 6.0;
-    c.a1_6h2 = a * (1.0 / (6.0 * h2));
-    c.a1_12 = a * (1.0 / 12.0);
-    c.ah2 = a / h2;
-}
-void coefs2d(Coefs<float>& c, float s, float a, float h2, float k)
-{
-    const double ad = a, h2d = h2, kd = k;
-    c = Coefs<float>{};
-    c.s = s; c.a = a; c.k = k;
-    c.l_a = (float)(1.0 / ad);
-    c.lh2 = (float)(1.0 / h2d);
-    c.k2 = (float)(kd / 2.0);
-    c.k6 = (float)(kd / 6.0);
-    c.a1_6h2 = (float)(ad * (1.0 / (6.0 * h2d)));
-    c.a1_12 = (float)(ad * (1.0 / 12.0));
-    c.ah2 = a / h2;
-}
-
-// NLSE3D_TAKE_STEPS_CD.c:304-306, NLSE3D_TAKE_STEPS_2SHOC.c:499-504 and the
-// float files' :315-317 / :509-514
-template <typename T> void coefs3d(Coefs<T>& c, T s, T a, T h2, T k)
-{
-    c = Coefs<T>{};
-    c.s = s; c.a = a; c.k = k;
-    c.ah2 = a / h2;
-    c.k2 = k / (T)2.0;
-    c.k6 = k / (T)6.0;
-    c.lh2 = (T)1.0 / h2;
-    c.a_6h2 = a / ((T)6.0 * h2);
-    c.a_12 = a / (T)12.0;
-    c.l_a = (T)1.0 / a;
-}
-
-template <typename T> __global__ void pack_kernel(const T* __restrict__ r, const T* __restrict__ i, c2<T>* __restrict__ u, long n)
-{
-    const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    if (p < n) u[p] = mk<T>(r[p], i[p]);
-}
-
-template <typename T> __global__ void unpack_kernel(const c2<T>* __restrict__ u, T* __restrict__ r, T* __restrict__ i, long n)
-{
-    const long p = (long)blockIdx.x * 256 + threadIdx.x;
-    if (p < n) {
-        const c2<T> v = u[p];
-        r[p] = v.x;
-        i[p] = v.y;
+    for (long p = 0; p < n; ++p) {
+        hi[p] = (float)x[p];
+        lo[p] = (float)(x[p] - (double)hi[p]);
     }
 }
+}  // namespace nlse
+
+namespace {
+
+int fail(int code, const std::string& msg) { return set_error(code, msg); }
 
 // plain streaming copy (16 B per lane), used to calibrate HBM byte counters
 __global__ __launch_bounds__(256) void copy_probe_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
@@ -122,6 +54,7 @@ struct nlse_solver {
     Coefs<double> cd;
     Coefs<float> cf;
     void *U, *Y0, *Y1, *acc, *D, *V;
+    void* Ulo;  // precision 3: low part of the compensated state
     hipStream_t stream;
     hipGraphExec_t gexec[2];  // per buffer holding U when the graph starts
     int gsteps;               // steps per graph replay (0: plain launches)
@@ -135,12 +68,15 @@ struct nlse_solver {
 namespace {
 
 size_t elem_bytes(int prec) { return prec == 2 ? 16 : 8; }
+// precision codes: 2 double, 1 single, 3 single with a compensated (hi + lo) state
+bool is_double(const nlse_solver* h) { return h->prec == 2; }
 
 int validate(int dim, int N, int M, int L, int method, int prec, int BC)
 {
     if (dim < 1 || dim > 3) return fail(NLSE_EINVAL, "dim must be 1, 2 or 3");
     if (method != 1 && method != 2) return fail(NLSE_EINVAL, "method must be 1 (CD) or 2 (2SHOC)");
-    if (prec != 1 && prec != 2) return fail(NLSE_EINVAL, "precision must be 1 (single) or 2 (double)");
+    if (prec < 1 || prec > 3)
+        return fail(NLSE_EINVAL, "precision must be 1 (single), 2 (double) or 3 (single, compensated)");
     if (N < 3 || (dim >= 2 && M < 3) || (dim == 3 && L < 3))
         return fail(NLSE_EINVAL, "every grid dimension must have at least 3 points");
     if (dim == 1 && BC == BC_ONESIDED && N < 4) return fail(NLSE_EINVAL, "BC=4 needs N >= 4");
@@ -192,6 +128,7 @@ int enqueue_stage_t(nlse_solver* h, const Coefs<T>& c, int stage)
     a.acc = (C*)h->acc;
     a.V = (const T*)h->V;
     a.dbuf = (C*)h->D;
+    a.ulo = (C*)h->Ulo;
     a.c = c;
     a.N = h->N; a.M = h->M; a.L = h->L;
     a.BC = h->BC; a.method = h->method;
@@ -209,7 +146,7 @@ int enqueue_stage_t(nlse_solver* h, const Coefs<T>& c, int stage)
 
 int enqueue_stage(nlse_solver* h, int stage)
 {
-    return h->prec == 2 ? enqueue_stage_t<double>(h, h->cd, stage) : enqueue_stage_t<float>(h, h->cf, stage);
+    return is_double(h) ? enqueue_stage_t<double>(h, h->cd, stage) : enqueue_stage_t<float>(h, h->cf, stage);
 }
 
 // launch l (0 <= l < 4/fusion) of one RK4 step; a fused step leaves the new
@@ -224,6 +161,7 @@ int enqueue_launch_t(nlse_solver* h, const Coefs<T>& c, int l)
     a.acc = (C*)h->acc;
     a.V = (const T*)h->V;
     a.dbuf = nullptr;
+    a.ulo = (C*)h->Ulo;
     a.c = c;
     a.N = h->N; a.M = h->M; a.L = h->L;
     a.BC = h->BC; a.method = h->method;
@@ -255,7 +193,7 @@ int enqueue_launch_t(nlse_solver* h, const Coefs<T>& c, int l)
 
 int enqueue_launch(nlse_solver* h, int l)
 {
-    return h->prec == 2 ? enqueue_launch_t<double>(h, h->cd, l) : enqueue_launch_t<float>(h, h->cf, l);
+    return is_double(h) ? enqueue_launch_t<double>(h, h->cd, l) : enqueue_launch_t<float>(h, h->cf, l);
 }
 
 template <typename T>
@@ -264,7 +202,8 @@ int enqueue_steps_t(nlse_solver* h, const Coefs<T>& c, int steps)
     using C = c2<T>;
     if (steps <= 0) return NLSE_OK;
     if (resident1d(h)) {
-        HIPCHK(launch_chunk1d_resident<T>((C*)h->U, (const T*)h->V, c, h->N, h->BC, h->method, steps, h->stream));
+        HIPCHK(launch_chunk1d_resident<T>((C*)h->U, (C*)h->Ulo, (const T*)h->V, c, h->N, h->BC, h->method, steps,
+                                          h->stream));
         return NLSE_OK;
     }
     const int nl = 4 / fusion(h);
@@ -278,7 +217,7 @@ int enqueue_steps_t(nlse_solver* h, const Coefs<T>& c, int steps)
 
 int enqueue_steps(nlse_solver* h, int steps)
 {
-    return h->prec == 2 ? enqueue_steps_t<double>(h, h->cd, steps) : enqueue_steps_t<float>(h, h->cf, steps);
+    return is_double(h) ? enqueue_steps_t<double>(h, h->cd, steps) : enqueue_steps_t<float>(h, h->cf, steps);
 }
 
 int step_impl(nlse_solver* h, int steps)
@@ -319,33 +258,49 @@ int step_impl(nlse_solver* h, int steps)
 template <typename T>
 int upload(nlse_solver* h, const T* Ur, const T* Ui, const T* V)
 {
-    const size_t b = sizeof(T) * (size_t)h->n;
-    T* stage_r = (T*)h->Y0;
-    T* stage_i = stage_r + h->n;
-    HIPCHK(hipMemcpyAsync(stage_r, Ur, b, hipMemcpyHostToDevice, h->stream));
-    if (Ui) HIPCHK(hipMemcpyAsync(stage_i, Ui, b, hipMemcpyHostToDevice, h->stream));
-    else HIPCHK(hipMemsetAsync(stage_i, 0, b, h->stream));
-    if (V) HIPCHK(hipMemcpyAsync(h->V, V, b, hipMemcpyHostToDevice, h->stream));
-    const unsigned nb = (unsigned)((h->n + 255) / 256);
-    hipLaunchKernelGGL((pack_kernel<T>), dim3(nb), dim3(256), 0, h->stream, stage_r, stage_i, (c2<T>*)h->U, h->n);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(h->stream));
-    return NLSE_OK;
+    if (V) HIPCHK(hipMemcpyAsync(h->V, V, sizeof(T) * (size_t)h->n, hipMemcpyHostToDevice, h->stream));
+    int rc = upload_planes<T>(Ur, Ui, (c2<T>*)h->U, (T*)h->Y0, h->n, h->stream);
+    if (rc == NLSE_OK && h->Ulo) HIPCHK(hipMemsetAsync(h->Ulo, 0, sizeof(c2<T>) * (size_t)h->n, h->stream));
+    if (rc == NLSE_OK) HIPCHK(hipStreamSynchronize(h->stream));
+    return rc;
 }
 
 template <typename T>
 int download(nlse_solver* h, T* Ur, T* Ui)
 {
-    const size_t b = sizeof(T) * (size_t)h->n;
-    T* stage_r = (T*)h->Y0;
-    T* stage_i = stage_r + h->n;
-    const unsigned nb = (unsigned)((h->n + 255) / 256);
-    hipLaunchKernelGGL((unpack_kernel<T>), dim3(nb), dim3(256), 0, h->stream, (const c2<T>*)h->U, stage_r, stage_i,
-                       h->n);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(Ur, stage_r, b, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(Ui, stage_i, b, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    return download_planes<T>((const c2<T>*)h->U, Ur, Ui, (T*)h->Y0, h->n, h->stream);
+}
+
+// double planes -> compensated single-precision state (hi, lo)
+int upload_comp(nlse_solver* h, const double* Ur, const double* Ui, const double* V)
+{
+    const long n = h->n;
+    std::vector<float> hr(n), hi(n), lr(n), li(n, 0.f), zero;
+    split_hilo(Ur, hr.data(), lr.data(), n);
+    if (Ui) split_hilo(Ui, hi.data(), li.data(), n);
+    else std::fill(hi.begin(), hi.end(), 0.f);
+    if (V) {
+        std::vector<float> v(n);
+        for (long p = 0; p < n; ++p) v[p] = (float)V[p];
+        HIPCHK(hipMemcpyAsync(h->V, v.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    int rc = upload_planes<float>(hr.data(), hi.data(), (float2*)h->U, (float*)h->Y0, n, h->stream);
+    if (rc == NLSE_OK) rc = upload_planes<float>(lr.data(), li.data(), (float2*)h->Ulo, (float*)h->Y0, n, h->stream);
+    return rc;
+}
+
+int download_comp(nlse_solver* h, double* Ur, double* Ui)
+{
+    const long n = h->n;
+    std::vector<float> hr(n), hi(n), lr(n), li(n);
+    int rc = download_planes<float>((const float2*)h->U, hr.data(), hi.data(), (float*)h->Y0, n, h->stream);
+    if (rc == NLSE_OK) rc = download_planes<float>((const float2*)h->Ulo, lr.data(), li.data(), (float*)h->Y0, n, h->stream);
+    if (rc != NLSE_OK) return rc;
+    for (long p = 0; p < n; ++p) {
+        Ur[p] = (double)hr[p] + (double)lr[p];
+        Ui[p] = (double)hi[p] + (double)li[p];
+    }
     return NLSE_OK;
 }
 
@@ -409,8 +364,10 @@ long long nlse_footprint_bytes(int dim, int N, int M, int L, int method, int pre
 {
     const long long n = (long long)N * (dim >= 2 ? M : 1) * (dim == 3 ? L : 1);
     const long long e = precision == 2 ? 16 : 8;
+    (void)method;
     long long b = 4 * n * e + n * (e / 2);  // U, Y0, Y1, ktot, V
     if (dim == 3 && method == 2) b += n * e;
+    if (precision == 3) b += n * e;  // low part of the compensated state
     return b;
 }
 
@@ -439,6 +396,10 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
         nlse_destroy(h);
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
+    if (precision == 3) {
+        if (hipMalloc(&h->Ulo, eb) != hipSuccess) { nlse_destroy(h); return fail(NLSE_ENOMEM, "hipMalloc failed"); }
+        (void)hipMemset(h->Ulo, 0, eb);
+    }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
         nlse_destroy(h);
@@ -461,7 +422,7 @@ void nlse_destroy(nlse_solver* h)
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
-    void* bufs[6] = {h->U, h->Y0, h->Y1, h->acc, h->D, h->V};
+    void* bufs[7] = {h->U, h->Y0, h->Y1, h->acc, h->D, h->V, h->Ulo};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -474,6 +435,7 @@ int nlse_set_state(nlse_solver* h, const double* Ur, const double* Ui, const dou
 {
     if (!h || !Ur) return fail(NLSE_EINVAL, "null argument");
     if (h->prec == 2) return upload<double>(h, Ur, Ui, V);
+    if (h->prec == 3) return upload_comp(h, Ur, Ui, V);
     std::vector<float> r(h->n), i(h->n, 0.0f), v;
     for (long p = 0; p < h->n; ++p) r[p] = (float)Ur[p];
     if (Ui) for (long p = 0; p < h->n; ++p) i[p] = (float)Ui[p];
@@ -484,7 +446,7 @@ int nlse_set_state(nlse_solver* h, const double* Ur, const double* Ui, const dou
 int nlse_set_state_f(nlse_solver* h, const float* Ur, const float* Ui, const float* V)
 {
     if (!h || !Ur) return fail(NLSE_EINVAL, "null argument");
-    if (h->prec != 1) return fail(NLSE_EINVAL, "nlse_set_state_f on a double-precision solver");
+    if (h->prec != 1) return fail(NLSE_EINVAL, "nlse_set_state_f needs a single-precision (1) solver");
     return upload<float>(h, Ur, Ui, V);
 }
 
@@ -492,6 +454,7 @@ int nlse_get_state(nlse_solver* h, double* Ur, double* Ui)
 {
     if (!h || !Ur || !Ui) return fail(NLSE_EINVAL, "null argument");
     if (h->prec == 2) return download<double>(h, Ur, Ui);
+    if (h->prec == 3) return download_comp(h, Ur, Ui);
     std::vector<float> r(h->n), i(h->n);
     int rc = download<float>(h, r.data(), i.data());
     if (rc != NLSE_OK) return rc;
@@ -502,7 +465,7 @@ int nlse_get_state(nlse_solver* h, double* Ur, double* Ui)
 int nlse_get_state_f(nlse_solver* h, float* Ur, float* Ui)
 {
     if (!h || !Ur || !Ui) return fail(NLSE_EINVAL, "null argument");
-    if (h->prec != 1) return fail(NLSE_EINVAL, "nlse_get_state_f on a double-precision solver");
+    if (h->prec != 1) return fail(NLSE_EINVAL, "nlse_get_state_f needs a single-precision (1) solver");
     return download<float>(h, Ur, Ui);
 }
 

@@ -34,15 +34,31 @@ template <typename T> struct Coefs {
 // NLSEmagic1D_CPU.c:33-76); any other value behaves like a zero boundary
 enum { BC_DIRICHLET = 1, BC_MSD = 2, BC_LAP0 = 3, BC_ONESIDED = 4 };
 
+// U += t for a state kept as an unevaluated sum hi + lo (compensated final
+// combine of the single-precision integrator): TwoSum(hi, t) [Knuth], the
+// rounding error folded into lo, then FastTwoSum renormalisation (|s| >= |l|).
+template <typename T> __device__ __forceinline__ void comp_add(T& hi, T& lo, T t)
+{
+    const T s = hi + t;
+    const T bb = s - hi;
+    const T e = (hi - (s - bb)) + (t - bb);
+    const T l = lo + e;
+    const T h = s + l;
+    lo = l - (h - s);
+    hi = h;
+}
+
 // RK4 stage index -> epilogue (NLSEmagic1D_CPU.c:405-418)
 //  1: ktot = K;          Y = U + k/2 K
 //  2: ktot = ktot + 2K;  Y = U + k/2 K
 //  3: ktot = ktot + 2K;  Y = U + k K
 //  4: ktot = ktot + 1K;  U = U + k/6 ktot
+// `ulo` (single precision only, else null): the low part of a compensated
+// state; stage 4 then adds k/6 ktot to hi + lo with comp_add.
 template <typename T>
 __device__ __forceinline__ void rk4_epilogue(int stage, const Coefs<T>& c, c2<T> f, c2<T> u,
                                              c2<T>* __restrict__ acc, c2<T>* __restrict__ yout,
-                                             c2<T>* __restrict__ uout, long p)
+                                             c2<T>* __restrict__ uout, long p, c2<T>* ulo = nullptr)
 {
     if (stage == 1) {
         acc[p] = f;
@@ -51,7 +67,15 @@ __device__ __forceinline__ void rk4_epilogue(int stage, const Coefs<T>& c, c2<T>
         c2<T> a = acc[p];
         a.x = a.x + (T)1.0 * f.x;
         a.y = a.y + (T)1.0 * f.y;
-        uout[p] = mk<T>(u.x + c.k6 * a.x, u.y + c.k6 * a.y);
+        if (ulo) {
+            c2<T> lo = ulo[p];
+            comp_add<T>(u.x, lo.x, c.k6 * a.x);
+            comp_add<T>(u.y, lo.y, c.k6 * a.y);
+            ulo[p] = lo;
+            uout[p] = u;
+        } else {
+            uout[p] = mk<T>(u.x + c.k6 * a.x, u.y + c.k6 * a.y);
+        }
     } else {
         c2<T> a = acc[p];
         a.x = a.x + (T)2.0 * f.x;

@@ -1,0 +1,129 @@
+// nlse_host.h -- host-side helpers shared by the C-ABI translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/nlsemagic_hip.h"
+#include "nlse_launch.h"
+
+namespace nlse {
+
+// record the message returned by nlse_last_error() (thread-local) and return code
+int set_error(int code, const std::string& msg);
+
+#define HIPCHK(x)                                                                                      \
+    do {                                                                                               \
+        hipError_t e_ = (x);                                                                           \
+        if (e_ != hipSuccess) return ::nlse::set_error(NLSE_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---- scalar precompute, in the precision and rounding of each reference file
+
+// NLSEmagic1D_CPU.c:384-390 (double) and :460-466 (float, all float arithmetic)
+template <typename T> inline void coefs1d(Coefs<T>& c, T s, T a, T h2, T k)
+{
+    c = Coefs<T>{};
+    c.s = s; c.a = a; c.k = k;
+    c.ah2 = a / h2;
+    c.k2 = k / (T)2.0;
+    c.k6 = k / (T)6.0;
+    c.l_a = (T)1.0 / a;
+    c.lh2 = (T)1.0 / h2;
+    c.a76 = a * ((T)7.0 / (T)6.0);
+    c.a112 = a * ((T)1.0 / (T)12.0);
+}
+
+// NLSEmagic2D_CPU.c:757-763 (double) and :865-871 (float: double expressions
+// rounded to float, except ah2 = a/h2, a float division)
+inline void coefs2d(Coefs<double>& c, double s, double a, double h2, double k)
+{
+    c = Coefs<double>{};
+    c.s = s; c.a = a; c.k = k;
+    c.l_a = 1.0 / a;
+    c.lh2 = 1.0 / h2;
+    c.k2 = k / 2.0;
+    c.k6 = k / 6.0;
+    c.a1_6h2 = a * (1.0 / (6.0 * h2));
+    c.a1_12 = a * (1.0 / 12.0);
+    c.ah2 = a / h2;
+}
+inline void coefs2d(Coefs<float>& c, float s, float a, float h2, float k)
+{
+    const double ad = a, h2d = h2, kd = k;
+    c = Coefs<float>{};
+    c.s = s; c.a = a; c.k = k;
+    c.l_a = (float)(1.0 / ad);
+    c.lh2 = (float)(1.0 / h2d);
+    c.k2 = (float)(kd / 2.0);
+    c.k6 = (float)(kd / 6.0);
+    c.a1_6h2 = (float)(ad * (1.0 / (6.0 * h2d)));
+    c.a1_12 = (float)(ad * (1.0 / 12.0));
+    c.ah2 = a / h2;
+}
+
+// NLSE3D_TAKE_STEPS_CD.c:304-306, NLSE3D_TAKE_STEPS_2SHOC.c:499-504 and the
+// float files' :315-317 / :509-514
+template <typename T> inline void coefs3d(Coefs<T>& c, T s, T a, T h2, T k)
+{
+    c = Coefs<T>{};
+    c.s = s; c.a = a; c.k = k;
+    c.ah2 = a / h2;
+    c.k2 = k / (T)2.0;
+    c.k6 = k / (T)6.0;
+    c.lh2 = (T)1.0 / h2;
+    c.a_6h2 = a / ((T)6.0 * h2);
+    c.a_12 = a / (T)12.0;
+    c.l_a = (T)1.0 / a;
+}
+
+
+// interleave two device planes into a complex array, and back
+template <typename T> __global__ void pack_kernel(const T* __restrict__ r, const T* __restrict__ i, c2<T>* __restrict__ u, long n)
+{
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) u[p] = mk<T>(r[p], i[p]);
+}
+
+template <typename T> __global__ void unpack_kernel(const c2<T>* __restrict__ u, T* __restrict__ r, T* __restrict__ i, long n)
+{
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    if (p < n) {
+        const c2<T> v = u[p];
+        r[p] = v.x;
+        i[p] = v.y;
+    }
+}
+
+// host planes -> interleaved device array `dst` of n points, through the
+// device staging area `stage` (>= 2n elements of T); Ui may be null (zero)
+template <typename T>
+int upload_planes(const T* Ur, const T* Ui, c2<T>* dst, T* stage, long n, hipStream_t st)
+{
+    const size_t b = sizeof(T) * (size_t)n;
+    HIPCHK(hipMemcpyAsync(stage, Ur, b, hipMemcpyHostToDevice, st));
+    if (Ui) HIPCHK(hipMemcpyAsync(stage + n, Ui, b, hipMemcpyHostToDevice, st));
+    else HIPCHK(hipMemsetAsync(stage + n, 0, b, st));
+    hipLaunchKernelGGL((pack_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, stage, stage + n, dst, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NLSE_OK;
+}
+
+template <typename T>
+int download_planes(const c2<T>* src, T* Ur, T* Ui, T* stage, long n, hipStream_t st)
+{
+    const size_t b = sizeof(T) * (size_t)n;
+    hipLaunchKernelGGL((unpack_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, stage, stage + n, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(Ur, stage, b, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(Ui, stage + n, b, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return NLSE_OK;
+}
+
+// split double planes into the hi + lo single-precision pair of a compensated state
+void split_hilo(const double* x, float* hi, float* lo, long n);
+
+}  // namespace nlse

@@ -16,6 +16,7 @@ template <typename T> struct StageArgs {
     c2<T>* yout;
     const T* V;
     c2<T>* dbuf;  // 3D 2SHOC only: D = Lap_CD(Y_in)/h^2 scratch, one per point
+    c2<T>* ulo;   // compensated single precision: low part of U (else null)
     Coefs<T> c;
     int N, M, L;  // 1D: N; 2D: N x M; 3D: L x N x M (M contiguous)
     int BC, method, stage;
@@ -32,9 +33,35 @@ template <typename T> hipError_t launch_stage3d(const StageArgs<T>& a, hipStream
 template <typename T> hipError_t launch_pair2d(const StageArgs<T>& a, bool first, hipStream_t st);
 template <typename T> hipError_t launch_step2d(const StageArgs<T>& a, hipStream_t st);
 
+// tiles of problem p are [tile0, tile0 + tiles_i*tiles_j); fields of every
+// per-point array of problem p start at element `off`
+template <typename T>
+struct BatchProb2 {
+    int N, M, BC;
+    int tiles_i, tiles_j, ioff, joff, tile0;
+    long off;
+    Coefs<T> c;
+};
+
+// One persistent launch of RK4 stages first..last over every tile of a batch
+// of 2D problems (tile heights 8, 16 or 32; `grid` workgroups, a multiple of 8).
+template <typename T> struct BatchArgs {
+    const c2<T>* yin;
+    const c2<T>* u;
+    c2<T>* acc;
+    c2<T>* out;
+    const T* V;
+    const BatchProb2<T>* probs;
+    const int* tile_prob;
+    int ntiles;
+    c2<T>* ulo;
+    int method, first, last, tile, grid;
+};
+template <typename T> hipError_t launch_batch2d(const BatchArgs<T>& a, hipStream_t st);
+
 // Whole chunk of a small 1D problem in one workgroup (state held on chip).
 template <typename T>
-hipError_t launch_chunk1d_resident(c2<T>* u, const T* V, Coefs<T> c, int N, int BC, int method,
+hipError_t launch_chunk1d_resident(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c, int N, int BC, int method,
                                    int steps, hipStream_t st);
 int chunk1d_resident_max_points(int elem_bytes);
 
