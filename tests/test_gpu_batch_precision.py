@@ -101,8 +101,9 @@ def test_compensated_single_precision_accuracy(hip, oracle, dim, shape, method):
     klin = (1.0 if method == 1 else 0.75) * h * h / (np.sqrt(2) * dim)
     k = 0.8 * klin
     steps = 300
-    BC = 2
+    BC = 2 if dim < 3 else 1  # the 3D case runs with Dirichlet boundaries, as the 3D workload does
     exp = oracle.take_steps(U, V, -1.0, 1.0, h * h, BC, steps, k, method)
+    assert np.all(np.isfinite(exp))
     got = {}
     for prec in (1, 3):
         sol = hip.Solver(shape, method, prec, BC, -1.0, 1.0, h * h, k)
