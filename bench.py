@@ -81,6 +81,100 @@ def cpu_baseline(U, V, k, budget_s=12.0):
             "sample": f"{steps} RK4 steps of the same 1024x1024 2SHOC FP64 problem ({dt:.1f} s, 1 thread)"}
 
 
+def batched_problems(nprob=512, seed=1):
+    """The batched workload: per problem, in this order from one PCG64 stream
+    seeded with 1: side n = round(exp(U[ln 64, ln 512])), s in {-1, +1},
+    a ~ U[0.5, 1], omega ~ U[0.05, 0.2].  Square n x n grid, h = 0.1, centred;
+    V = -omega^2 (x^2 + y^2); U0 = exp(-r^2/2) (s = +1) or tanh(r) e^{i theta}
+    (s = -1); k = 0.8 * (3/4) h^2 / (2 sqrt 2 a) (NLSEmagic2D.m:129-136)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nprob):
+        n = int(round(np.exp(rng.uniform(np.log(64), np.log(512)))))
+        s = float(rng.choice([-1.0, 1.0]))
+        a = float(rng.uniform(0.5, 1.0))
+        w = float(rng.uniform(0.05, 0.2))
+        out.append(dict(n=n, s=s, a=a, w=w, k=0.8 * 0.75 * H * H / (2 * np.sqrt(2) * a)))
+    return out
+
+
+def batched_state(p):
+    n = p["n"]
+    x = H * (np.arange(n) - n / 2)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    r2 = X * X + Y * Y
+    V = -p["w"] ** 2 * r2
+    if p["s"] > 0:
+        U = np.exp(-r2 / 2) + 0j
+    else:
+        U = np.tanh(np.sqrt(r2)) * np.exp(1j * np.arctan2(Y, X))
+    return U, V
+
+
+def run_batched(args):
+    """512 uneven 2D 2SHOC problems, chunks of 100 steps, FP64 and FP32c."""
+    import nlsemagic_amd as nm
+    probs = batched_problems()
+    shapes = [(p["n"], p["n"]) for p in probs]
+    npts = sum(p["n"] ** 2 for p in probs)
+    chunk = 100
+    nchunks = args.steps
+    res = {}
+    finals = {}
+    for prec in (2, 3):
+        B = nm.Batch(shapes, [p["s"] for p in probs], [p["a"] for p in probs], [H * H] * len(probs),
+                     [p["k"] for p in probs], method=2, precision=prec, BC=1)
+        states = [batched_state(p) for p in probs]
+        for i, (U, V) in enumerate(states):
+            B.set_state(i, U, V)
+        f, t, g, ms_tune = B.autotune(3)
+        B.configure(f, t, g, chunk)
+        B.step(chunk)  # warm-up: captures the chunk graph
+        B.sync()
+        for i, (U, V) in enumerate(states):
+            B.set_state(i, U)
+        lat = []
+        for _ in range(nchunks):
+            lat.append(B.step_timed(chunk))
+        diag = B.diagnostics()
+        finals[prec] = [B.state(i) for i in range(len(probs))] if prec == 3 or True else None
+        m0 = np.array([np.sum(np.abs(U) ** 2) for U, _ in states])
+        res[prec] = dict(rate=npts * chunk * nchunks / (sum(lat) * 1e-3), chunk_ms=lat, fusion=f, tile=t, grid=g,
+                         mass=diag[:, 0], mass_drift=diag[:, 0] / m0 - 1, nonfinite=int(diag[:, 2].sum()))
+        B.close()
+    # FP32c vs FP64 per problem (the FP64 batch is the FP64 reference at full size)
+    rel = np.array([np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in zip(finals[3], finals[2])])
+    mass_rel = res[3]["mass"] / res[2]["mass"] - 1
+    # pin the full-size FP64 reference: the smallest problems against the CPU oracle, bit for bit
+    from oracle import oracle as O
+    order = np.argsort([p["n"] for p in probs])[:3]
+    pinned = 0
+    for i in order:
+        p = probs[i]
+        U, V = batched_state(p)
+        exp = O.take_steps(U, V, p["s"], p["a"], H * H, 1, chunk * nchunks, p["k"], 2)
+        pinned += int(np.array_equal(exp.view(np.uint64), finals[2][i].view(np.uint64)))
+    pct = lambda x, q: float(np.percentile(x, q))
+    line = {
+        "metric": "RK4 grid-point updates/s & % HBM peak; batched-solve p50/p99 latency",
+        "value": res[3]["rate"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": nchunks, "warmup": 1,
+        "ms_per_step": float(np.mean(res[3]["chunk_ms"])), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (compensated hi+lo state)", "data": "synthetic",
+        "config": {"workload": "batched: 512 uneven 2D 2SHOC problems (sides 64-512, seed 1), chunks of 100 steps",
+                   "points": npts, "fp64_rate": res[2]["rate"], "fp32c_over_fp64": res[3]["rate"] / res[2]["rate"],
+                   "fp32c_rel_l2_max": float(rel.max()), "fp32c_rel_l2_p50": pct(rel, 50),
+                   "fp32c_mass_vs_fp64_max": float(np.abs(mass_rel).max()),
+                   "fp64_mass_drift_max": float(np.abs(res[2]["mass_drift"]).max()),
+                   "fp32c_mass_drift_max": float(np.abs(res[3]["mass_drift"]).max()),
+                   "nonfinite": res[2]["nonfinite"] + res[3]["nonfinite"],
+                   "chunk_ms_p50": {k: pct(v["chunk_ms"], 50) for k, v in (("fp64", res[2]), ("fp32c", res[3]))},
+                   "chunk_ms_p99": {k: pct(v["chunk_ms"], 99) for k, v in (("fp64", res[2]), ("fp32c", res[3]))},
+                   "tuned": {k: (v["fusion"], v["tile"], v["grid"]) for k, v in (("fp64", res[2]), ("fp32c", res[3]))},
+                   "fp64_pinned_to_oracle": f"{pinned}/3 smallest problems bit-identical after {chunk * nchunks} steps"},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +186,8 @@ def main():
                     help="RK4 stages per kernel launch (results are identical)")
     ap.add_argument("--tile", type=int, default=8, choices=(8, 16, 32), help="tile height in rows")
     ap.add_argument("--autotune", action="store_true", help="pick fusion and tile with the built-in tuner")
+    ap.add_argument("--workload", default="2d", choices=("2d", "batched"),
+                    help="2d: the 1024^2 FP64 2SHOC headline; batched: 512 uneven problems, FP32c vs FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -105,6 +201,11 @@ def main():
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "the integrator does not shard a grid across GPUs", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        if args.steps == ap.get_default("steps"):
+            args.steps = 10
+        run_batched(args)
+        return
 
     import nlsemagic_amd as nm
 

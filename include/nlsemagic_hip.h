@@ -104,6 +104,10 @@ int nlse_step(nlse_solver *h, int steps);
  * wait, and return the elapsed device time in milliseconds */
 int nlse_step_timed(nlse_solver *h, int steps, double *ms);
 int nlse_sync(nlse_solver *h);
+/* diagnostics of the resident state, reduced on the GPU in double precision:
+ * out3[0] = sum |U|^2 (mass / h^d), out3[1] = max Re U, out3[2] = number of
+ * non-finite points (the reference drivers' blow-up test) */
+int nlse_diagnostics(nlse_solver *h, double *out3);
 /* run `steps` steps with a HIP event around every kernel launch and return
  * the mean device time of each launch of a step in ms_launch[0..nlaunch-1]
  * (nlaunch = 4 / stages per launch; per-kernel timing for roofline reports;
@@ -143,6 +147,8 @@ int nlse_batch_get_state(nlse_batch *b, int p, double *Ur, double *Ui);
 int nlse_batch_step(nlse_batch *b, int steps);
 int nlse_batch_step_timed(nlse_batch *b, int steps, double *ms);
 int nlse_batch_sync(nlse_batch *b);
+/* per-problem diagnostics, out[3p..3p+2] as for nlse_diagnostics */
+int nlse_batch_diagnostics(nlse_batch *b, double *out);
 /* stages per launch (1, 2, 4), tile rows (8, 16, 32), persistent grid
  * (multiple of 8 workgroups) and hipGraph steps per replay (0 = off) */
 int nlse_batch_configure(nlse_batch *b, int stages_per_launch, int tile_rows, int grid,

@@ -112,6 +112,10 @@ def lib():
     L.nlse_batch_configure.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
     L.nlse_batch_autotune.restype = C.c_int
     L.nlse_batch_autotune.argtypes = [C.c_void_p, C.c_int, _I, _I, _I, C.POINTER(C.c_double)]
+    L.nlse_diagnostics.restype = C.c_int
+    L.nlse_diagnostics.argtypes = [C.c_void_p, _D]
+    L.nlse_batch_diagnostics.restype = C.c_int
+    L.nlse_batch_diagnostics.argtypes = [C.c_void_p, _D]
     L.nlse_copy_probe.restype = C.c_int
     L.nlse_copy_probe.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.nlse_footprint_bytes.restype = C.c_longlong
@@ -287,6 +291,12 @@ class Solver:
         _check(self._lib.nlse_step_timed(self._h, int(steps), C.byref(ms)))
         return ms.value
 
+    def diagnostics(self):
+        """(sum |U|^2, max Re U, non-finite count), reduced on the GPU in FP64."""
+        out = np.empty(3)
+        _check(self._lib.nlse_diagnostics(self._h, _p(out, C.c_double)))
+        return float(out[0]), float(out[1]), int(out[2])
+
     def profile_launches(self, steps):
         """Mean device ms of each kernel launch of an RK4 step (HIP events)."""
         out = (C.c_double * 4)()
@@ -381,6 +391,12 @@ class Batch:
 
     def sync(self):
         _check(self._lib.nlse_batch_sync(self._h))
+
+    def diagnostics(self):
+        """Per-problem (sum |U|^2, max Re U, non-finite count) as an (nprob, 3) array."""
+        out = np.empty((len(self.shapes), 3))
+        _check(self._lib.nlse_batch_diagnostics(self._h, _p(out, C.c_double)))
+        return out
 
     def configure(self, stages_per_launch=2, tile_rows=8, grid=1024, steps_per_graph=0):
         _check(self._lib.nlse_batch_configure(self._h, int(stages_per_launch), int(tile_rows), int(grid),

@@ -322,6 +322,28 @@ int nlse_batch_step_timed(nlse_batch* b, int steps, double* ms)
     return NLSE_OK;
 }
 
+int nlse_batch_diagnostics(nlse_batch* b, double* out)
+{
+    if (!b || !out) return fail(NLSE_EINVAL, "null argument");
+    double* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, sizeof(double) * (3 * 1024 + 3 * (size_t)b->nprob)));
+    double* part = d + 3 * (size_t)b->nprob;
+    const size_t eb = b->prec == 2 ? 16 : 8;
+    hipError_t e = hipSuccess;
+    for (int p = 0; p < b->nprob && e == hipSuccess; ++p) {
+        const long n = (long)b->N[p] * b->M[p];
+        const char* u = (const char*)b->U + eb * b->off[p];
+        const char* lo = b->Ulo ? (const char*)b->Ulo + eb * b->off[p] : nullptr;
+        e = launch_diag(u, lo, b->prec == 2, n, part, d + 3 * p, b->stream);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d, 3 * sizeof(double) * b->nprob, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("diagnostics: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
+
 int nlse_batch_sync(nlse_batch* b)
 {
     if (!b) return fail(NLSE_EINVAL, "null batch");

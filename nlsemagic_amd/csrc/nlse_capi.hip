@@ -520,6 +520,19 @@ int nlse_profile_launches(nlse_solver* h, int steps, double* ms_launch, int* nla
     return NLSE_OK;
 }
 
+int nlse_diagnostics(nlse_solver* h, double* out3)
+{
+    if (!h || !out3) return fail(NLSE_EINVAL, "null argument");
+    double* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, sizeof(double) * (3 * 1024 + 3)));
+    hipError_t e = launch_diag(h->U, h->Ulo, is_double(h), h->n, d + 3, d, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out3, d, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("diagnostics: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
+
 int nlse_sync(nlse_solver* h)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "../../include/nlsemagic_hip.h"
@@ -122,6 +123,10 @@ int download_planes(const c2<T>* src, T* Ur, T* Ui, T* stage, long n, hipStream_
     HIPCHK(hipStreamSynchronize(st));
     return NLSE_OK;
 }
+
+// diagnostics of n points (u, plus ulo for a compensated state) into out[0..2]
+// on the device: sum |U|^2, max Re U, non-finite count; part: 3*1024 doubles
+hipError_t launch_diag(const void* u, const void* ulo, bool dbl, long n, double* part, double* out, hipStream_t st);
 
 // split double planes into the hi + lo single-precision pair of a compensated state
 void split_hilo(const double* x, float* hi, float* lo, long n);

@@ -165,3 +165,49 @@ def test_solver_autotune_keeps_state(hip, oracle):
     sol.step(4)
     assert bits_equal(sol.state(), oracle.take_steps(U, V, -1.0, 1.0, 0.01, 2, 4, 0.002, 2))
     sol.close()
+
+
+@pytest.mark.parametrize("prec", [2, 3])
+def test_device_diagnostics(hip, prec):
+    """FP64 on-device mass / max Re / non-finite count match numpy on the state."""
+    rng = np.random.default_rng(12)
+    shape = (130, 70)
+    U = rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+    sol = hip.Solver(shape, 2, prec, 1, -1.0, 1.0, 0.01, 0.001)
+    sol.set_state(U, np.zeros(shape))
+    sol.step(3)
+    W = sol.state()
+    mass, mxr, bad = sol.diagnostics()
+    assert bad == 0
+    assert abs(mass / np.sum(np.abs(W) ** 2) - 1) < 1e-13
+    assert mxr == np.max(W.real) or abs(mxr - np.max(W.real)) < 1e-15
+    sol.close()
+    probs = _batch_inputs(4, 2)
+    B = hip.Batch([p["shape"] for p in probs], [p["s"] for p in probs], [p["a"] for p in probs],
+                  [p["h2"] for p in probs], [p["k"] for p in probs], method=2, precision=prec, BC=1)
+    for i, p in enumerate(probs):
+        B.set_state(i, p["U"], p["V"])
+    B.step(2)
+    D = B.diagnostics()
+    for i in range(len(probs)):
+        W = B.state(i)
+        assert abs(D[i, 0] / np.sum(np.abs(W) ** 2) - 1) < 1e-13
+        assert D[i, 2] == 0
+    B.close()
+
+
+def test_blowup_is_counted(hip):
+    """2SHOC at the 2D workload's k = 0.3 h^2 exceeds RK4's stability limit:
+    the state overflows and the non-finite count reports it."""
+    n = 128
+    h = 0.1
+    x = h * (np.arange(n) - n / 2)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    rng = np.random.default_rng(0)
+    U = np.tanh(np.hypot(X, Y)) * np.exp(1j * np.arctan2(Y, X)) + 1e-3 * rng.uniform(-1, 1, X.shape)
+    sol = hip.Solver((n, n), 2, 2, 2, -1.0, 1.0, h * h, 0.3 * h * h)
+    sol.set_state(U, np.zeros((n, n)))
+    sol.step(400)
+    mass, mxr, bad = sol.diagnostics()
+    assert bad > 0
+    sol.close()
