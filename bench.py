@@ -24,6 +24,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -175,6 +176,136 @@ def run_batched(args):
     print(json.dumps(line))
 
 
+def vortex_rings_3d(n=256, L=25.6, nrings=4, radius=8.0, seed=0):
+    """Product of dark vortex rings tanh(d/sqrt 2) e^{+-i phi} on [-L, L)^3:
+    d is the distance to the ring core, phi the poloidal angle around it.
+    Per ring, from one PCG64 stream seeded with `seed`: the unit normal
+    (three normal draws, normalised), the centre (three uniform draws in
+    [-6, 6]) and the winding (+1 or -1).  Returns U with shape (n, n, n)."""
+    rng = np.random.default_rng(seed)
+    h = 2 * L / n
+    x = -L + h * np.arange(n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    U = np.ones((n, n, n), complex)
+    for _ in range(nrings):
+        nv = rng.normal(size=3)
+        nv /= np.linalg.norm(nv)
+        c = rng.uniform(-6, 6, 3)
+        w = float(rng.choice([-1.0, 1.0]))
+        vx, vy, vz = X - c[0], Y - c[1], Z - c[2]
+        z = vx * nv[0] + vy * nv[1] + vz * nv[2]
+        rho = np.sqrt(np.maximum(vx * vx + vy * vy + vz * vz - z * z, 0.0))
+        d = np.sqrt((rho - radius) ** 2 + z * z)
+        U *= np.tanh(d / np.sqrt(2)) * np.exp(1j * w * np.arctan2(z, rho - radius))
+    return U, h
+
+
+def run_grid(args):
+    """1D (N = 2048, CD, MSD, dark soliton) or 3D (256^3, Dirichlet, vortex
+    rings) single-problem workloads: device rate for FP64 / FP32 / FP32c and
+    the CPU integrator on a bounded sample."""
+    import nlsemagic_amd as nm
+    from oracle import oracle as O
+    out = {}
+    if args.workload == "1d":
+        n = 2048
+        h = 100.0 / n
+        x = -50 + h * np.arange(n)
+        rng = np.random.default_rng(0)
+        U = np.tanh(x / np.sqrt(2)) + 1e-3 * (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n))
+        V = np.zeros(n)
+        shape, BC, methods, chunk, nchunks = (n,), 2, (1,), 100, 100
+        ks = {1: 0.4 * h * h}
+        desc = "1D CD RK4, N=2048 on [-50,50), MSD, dark soliton + 1e-3 noise (seed 0), 100 chunks of 100 steps"
+    else:
+        U, h = vortex_rings_3d()
+        V = np.zeros(U.shape)
+        shape, BC, methods, chunk, nchunks = U.shape, 1, (1, 2), 50, 20
+        ks = {1: 0.8 * h * h / (3 * np.sqrt(2)), 2: 0.8 * 0.75 * h * h / (3 * np.sqrt(2))}
+        desc = "3D RK4, 256^3 on [-25.6,25.6)^3, Dirichlet, four vortex rings (seed 0), 20 chunks of 50 steps"
+    npts = int(np.prod(shape))
+    for m in methods:
+        for prec in (2, 1, 3):
+            sol = nm.Solver(shape, m, prec, BC, -1.0, 1.0, h * h, ks[m])
+            sol.set_state(U, V)
+            sol.step(chunk)
+            sol.sync()
+            sol.set_state(U)
+            lat = [sol.step_timed(chunk) for _ in range(nchunks)]
+            mass, _, bad = sol.diagnostics()
+            W = sol.state() if prec == 2 or args.workload == "1d" else None
+            out[(m, prec)] = dict(rate=npts * chunk * nchunks / (sum(lat) * 1e-3), chunk_ms=float(np.median(lat)),
+                                  mass=mass, bad=bad, W=W)
+            sol.close()
+    # CPU integrator on a bounded sample (1 thread): the reference for 1D, the restatement for 3D
+    use_ref = args.workload == "1d" and O.ref(1) is not None
+    cpu = {}
+    for m in methods:
+        steps = 2000 if args.workload == "1d" else 2
+        t0 = time.perf_counter()
+        O.take_steps(U, V, -1.0, 1.0, h * h, BC, steps, ks[m], m, use_ref=use_ref)
+        cpu[m] = npts * steps / (time.perf_counter() - t0)
+    m0 = float(np.sum(np.abs(U) ** 2))
+    fp64 = {m: out[(m, 2)] for m in methods}
+    line = {
+        "metric": "RK4 grid-point updates/s & % HBM peak; batched-solve p50/p99 latency",
+        "value": out[(methods[-1], 2)]["rate"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": nchunks,
+        "warmup": 1, "ms_per_step": out[(methods[-1], 2)]["chunk_ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": desc,
+                   "rates": {f"{'CD' if m == 1 else '2SHOC'}_{ {2: 'fp64', 1: 'fp32', 3: 'fp32c'}[p] }": v["rate"]
+                             for (m, p), v in out.items()},
+                   "mass_drift_fp64": {('CD' if m == 1 else '2SHOC'): fp64[m]["mass"] / m0 - 1 for m in methods},
+                   "nonfinite": int(sum(v["bad"] for v in out.values())),
+                   "cpu_rate_1thread": {('CD' if m == 1 else '2SHOC'): cpu[m] for m in methods},
+                   "cpu_kind": "reference" if use_ref else "port"},
+    }
+    if args.workload == "1d":
+        exp = O.take_steps(U, V, -1.0, 1.0, h * h, BC, chunk * (nchunks + 0), ks[1], 1)
+        line["config"]["fp64_bit_identical_to_oracle"] = bool(
+            np.array_equal(exp.view(np.uint64), out[(1, 2)]["W"].view(np.uint64)))
+        line["config"]["fp32c_rel_l2"] = float(np.linalg.norm(out[(1, 3)]["W"] - exp) / np.linalg.norm(exp))
+    print(json.dumps(line))
+
+
+def run_serving(args):
+    """Poisson request stream (seed 2) through the continuous-batching server,
+    then the same requests one at a time through the plain FP64 path."""
+    import nlsemagic_amd as nm
+    from nlsemagic_amd import serving as S
+    warm = S.request_mix(12, qps=1000.0, seed=99)  # loads every kernel variant before timing
+    for r in warm:
+        r.steps = 20
+        r.shape = tuple(min(n, 64) for n in r.shape)
+        r.make_state = S._dark_state(r.shape, 0.1)
+    S.Server(max_active=8, chunk_steps=10).run(warm)
+    reqs = S.request_mix(args.requests, qps=args.qps, seed=2)
+    srv = S.Server(budget_bytes=200e9, max_active=16, chunk_steps=200)
+    stats = srv.run(reqs)
+    served = {r.rid: r.result for r in reqs}
+    lat_tagged = {p: [r.latency for r in reqs if r.precision == p] for p in (2, 3)}
+    base = [S.Request(**{f.name: getattr(r, f.name) for f in dataclasses.fields(S.Request)
+                         if f.name not in ("admitted", "finished", "result")}) for r in reqs]
+    plain = S.run_one_at_a_time(base, precision=2)
+    same = sum(int(np.array_equal(served[b.rid].view(np.uint64), b.result.view(np.uint64)))
+               for b in base if b.precision == 2)
+    n64 = sum(1 for b in base if b.precision == 2)
+    line = {
+        "metric": "RK4 grid-point updates/s & % HBM peak; batched-solve p50/p99 latency",
+        "value": stats["throughput"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": len(reqs), "warmup": 1,
+        "ms_per_step": stats["makespan_s"] * 1e3 / len(reqs), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 + f32c (per request)", "data": "synthetic",
+        "config": {"workload": f"serving: {len(reqs)} Poisson requests at {args.qps} QPS (seed 2), mixed 1D/2D/3D "
+                               "CD/2SHOC FP64/FP32c, 100-20000 steps, 200 GB budget",
+                   "server": stats, "one_at_a_time_fp64": plain,
+                   "latency_p50_s": stats["latency_p50_s"], "latency_p99_s": stats["latency_p99_s"],
+                   "latency_p50_by_precision": {("fp64" if p == 2 else "fp32c"): float(np.percentile(v, 50))
+                                                for p, v in lat_tagged.items() if v},
+                   "fp64_requests_identical_to_one_at_a_time": f"{same}/{n64}"},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,7 +317,9 @@ def main():
                     help="RK4 stages per kernel launch (results are identical)")
     ap.add_argument("--tile", type=int, default=8, choices=(8, 16, 32), help="tile height in rows")
     ap.add_argument("--autotune", action="store_true", help="pick fusion and tile with the built-in tuner")
-    ap.add_argument("--workload", default="2d", choices=("2d", "batched"),
+    ap.add_argument("--qps", type=float, default=4.0, help="serving: Poisson arrival rate")
+    ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
+    ap.add_argument("--workload", default="2d", choices=("2d", "batched", "1d", "3d", "serving"),
                     help="2d: the 1024^2 FP64 2SHOC headline; batched: 512 uneven problems, FP32c vs FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -201,6 +334,12 @@ def main():
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
               "the integrator does not shard a grid across GPUs", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serving":
+        run_serving(args)
+        return
+    if args.workload in ("1d", "3d"):
+        run_grid(args)
+        return
     if args.workload == "batched":
         if args.steps == ap.get_default("steps"):
             args.steps = 10

@@ -104,6 +104,9 @@ int nlse_step(nlse_solver *h, int steps);
  * wait, and return the elapsed device time in milliseconds */
 int nlse_step_timed(nlse_solver *h, int steps, double *ms);
 int nlse_sync(nlse_solver *h);
+/* non-blocking: 1 when all work enqueued on the solver's stream is done,
+ * 0 while it runs, a negative NLSE_E* code on error */
+int nlse_query(nlse_solver *h);
 /* diagnostics of the resident state, reduced on the GPU in double precision:
  * out3[0] = sum |U|^2 (mass / h^d), out3[1] = max Re U, out3[2] = number of
  * non-finite points (the reference drivers' blow-up test) */

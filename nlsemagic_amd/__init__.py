@@ -88,6 +88,8 @@ def lib():
     L.nlse_set_fusion.argtypes = [C.c_void_p, C.c_int]
     L.nlse_sync.restype = C.c_int
     L.nlse_sync.argtypes = [C.c_void_p]
+    L.nlse_query.restype = C.c_int
+    L.nlse_query.argtypes = [C.c_void_p]
     L.nlse_set_graph.restype = C.c_int
     L.nlse_set_graph.argtypes = [C.c_void_p, C.c_int]
     _I = C.POINTER(C.c_int)
@@ -324,6 +326,13 @@ class Solver:
     def sync(self):
         _check(self._lib.nlse_sync(self._h))
 
+    def done(self):
+        """True when every step enqueued so far has finished (non-blocking)."""
+        rc = self._lib.nlse_query(self._h)
+        if rc < 0:
+            _check(-rc)
+        return rc == 1
+
     def set_graph(self, steps_per_graph):
         _check(self._lib.nlse_set_graph(self._h, int(steps_per_graph)))
 
@@ -415,6 +424,12 @@ def copy_probe(nbytes, reps=10):
     ms = C.c_double()
     _check(lib().nlse_copy_probe(int(nbytes), int(reps), C.byref(ms)))
     return ms.value
+
+
+def footprint_bytes(shape, method, precision):
+    """Device bytes a Solver of this shape holds (admission control)."""
+    t = list(_tail(tuple(shape))) + [1, 1]
+    return int(lib().nlse_footprint_bytes(len(shape), t[0], t[1], t[2], int(method), int(precision)))
 
 
 def device_count():

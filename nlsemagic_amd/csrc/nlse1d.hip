@@ -151,8 +151,10 @@ hipError_t launch_stage1d(const StageArgs<T>& a, hipStream_t st)
 
 // ------------------------------------------------ whole chunk, one workgroup
 
-constexpr int RES_THREADS = 256;
-constexpr int RES_PPT = 16;  // points per thread -> N <= 4096
+// 1024 threads: four waves on each SIMD of the CU hide the LDS and barrier
+// latency of the stage-by-stage exchange
+constexpr int RES_THREADS = 1024;
+constexpr int RES_PPT = 4;  // points per thread -> N <= 4096
 
 int chunk1d_resident_max_points(int elem_bytes) { (void)elem_bytes; return RES_THREADS * RES_PPT; }
 
@@ -249,9 +251,8 @@ template <typename T>
 hipError_t launch_chunk1d_resident(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c, int N, int BC, int method,
                                    int steps, hipStream_t st)
 {
+    if (N <= RES_THREADS) return launch_res<T, 1>(u, ulo, V, c, N, BC, method, steps, st);
     if (N <= RES_THREADS * 2) return launch_res<T, 2>(u, ulo, V, c, N, BC, method, steps, st);
-    if (N <= RES_THREADS * 4) return launch_res<T, 4>(u, ulo, V, c, N, BC, method, steps, st);
-    if (N <= RES_THREADS * 8) return launch_res<T, 8>(u, ulo, V, c, N, BC, method, steps, st);
     return launch_res<T, RES_PPT>(u, ulo, V, c, N, BC, method, steps, st);
 }
 

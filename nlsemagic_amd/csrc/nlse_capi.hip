@@ -533,6 +533,15 @@ int nlse_diagnostics(nlse_solver* h, double* out3)
     return NLSE_OK;
 }
 
+int nlse_query(nlse_solver* h)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    const hipError_t e = hipStreamQuery(h->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return -fail(NLSE_EHIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
 int nlse_sync(nlse_solver* h)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");

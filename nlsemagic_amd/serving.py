@@ -1,0 +1,249 @@
+"""Continuous-batching request loop over device-resident NLSE solvers.
+
+A request is one chunked solve (dimension, grid, CD/2SHOC, FP64 or
+compensated FP32, step count, initial state and potential).  The server:
+
+- admits arrived requests in arrival order, with backfill, while the sum of
+  their device footprints (nlse_footprint_bytes) stays under the memory
+  budget and fewer than ``max_active`` are running;
+- keeps every admitted request's U, V and RK4 work arrays resident on the
+  GPU until it finishes (one resident solver per request, each on its own
+  HIP stream, so the work of different requests overlaps on the device);
+- advances each active request by chunks of ``chunk_steps`` RK4 steps, one
+  hipGraph replay per chunk, and enqueues the next chunk of a request as soon
+  as its previous one completes (non-blocking stream queries), so requests
+  that arrive while others run join the running mix without waiting for it
+  to drain;
+- releases a finished request's memory at once and records its latency
+  (finish - arrival).
+
+Each request's result is bit-for-bit the result of running it alone: the
+kernels are deterministic and requests share no state.
+
+The reference has no serving layer; its drivers call one MEX integrator per
+chunk in a loop (src/NLSEmagic2D/NLSEmagic2D.m:251-289).
+"""
+from __future__ import annotations
+
+import dataclasses
+import time
+from collections import deque
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from . import Solver, footprint_bytes
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float            # seconds after the start of the run
+    shape: tuple              # (N,), (N, M) or (L, N, M)
+    method: int               # 1 CD, 2 2SHOC
+    precision: int            # 2 double, 3 compensated single
+    BC: int
+    s: float
+    a: float
+    h2: float
+    k: float
+    steps: int
+    make_state: Callable[[], tuple]  # -> (U0, V)
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    result: Optional[np.ndarray] = None
+
+    @property
+    def points(self) -> int:
+        return int(np.prod(self.shape))
+
+    @property
+    def work(self) -> int:
+        """grid-point RK4 steps"""
+        return self.points * self.steps
+
+    @property
+    def latency(self) -> float:
+        return self.finished - self.arrival
+
+
+class Server:
+    def __init__(self, budget_bytes: float = 200e9, max_active: int = 16, chunk_steps: int = 200,
+                 keep_results: bool = True):
+        self.budget = float(budget_bytes)
+        self.max_active = int(max_active)
+        self.chunk = int(chunk_steps)
+        self.keep_results = keep_results
+        self.peak_bytes = 0
+        self.peak_active = 0
+
+    def _admit(self, r: Request, now: float):
+        sol = Solver(r.shape, r.method, r.precision, r.BC, r.s, r.a, r.h2, r.k)
+        sol.set_graph(min(self.chunk, r.steps))
+        U0, V = r.make_state()
+        sol.set_state(U0, V)
+        r.admitted = now
+        return sol
+
+    def run(self, requests: List[Request]) -> dict:
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        waiting: List[Request] = []
+        active = {}  # rid -> [request, solver, steps_left, footprint]
+        used = 0
+        t0 = time.perf_counter()
+        clock = lambda: time.perf_counter() - t0
+        while pending or waiting or active:
+            now = clock()
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            # admission: arrival order, later requests may backfill around one that does not fit
+            still = []
+            for r in waiting:
+                fp = footprint_bytes(r.shape, r.method, r.precision)
+                if len(active) < self.max_active and used + fp <= self.budget:
+                    active[r.rid] = [r, self._admit(r, clock()), r.steps, fp]
+                    used += fp
+                else:
+                    still.append(r)
+            waiting = still
+            self.peak_bytes = max(self.peak_bytes, used)
+            self.peak_active = max(self.peak_active, len(active))
+            progressed = False
+            for rid in list(active):
+                r, sol, left, fp = active[rid]
+                if not sol.done():
+                    continue
+                progressed = True
+                if left == 0:
+                    if self.keep_results:
+                        r.result = sol.state()
+                    else:
+                        sol.sync()
+                    r.finished = clock()
+                    sol.close()
+                    used -= fp
+                    del active[rid]
+                    continue
+                n = min(self.chunk, left)
+                sol.step(n)
+                active[rid][2] = left - n
+            if not progressed:
+                if not active and pending and not waiting:
+                    time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
+                else:
+                    time.sleep(50e-6)
+        makespan = clock()
+        return summarize(requests, makespan, peak_bytes=self.peak_bytes, peak_active=self.peak_active)
+
+
+def run_one_at_a_time(requests: List[Request], precision: Optional[int] = 2) -> dict:
+    """The plain path: each request solved alone through the drop-in entry
+    point (upload, all steps, download), in arrival order, one at a time.
+    ``precision`` forces every request to that precision (None: keep tags)."""
+    from . import take_steps
+    t0 = time.perf_counter()
+    for r in sorted(requests, key=lambda r: r.arrival):
+        while time.perf_counter() - t0 < r.arrival:
+            time.sleep(min(0.01, r.arrival - (time.perf_counter() - t0)))
+        r.admitted = time.perf_counter() - t0
+        U0, V = r.make_state()
+        prec = r.precision if precision is None else precision
+        if prec == 3:
+            prec = 1
+        r.result = take_steps(U0, V, r.s, r.a, r.h2, r.BC, r.steps, r.k, precision=prec, method=r.method)
+        r.finished = time.perf_counter() - t0
+    return summarize(requests, time.perf_counter() - t0)
+
+
+def summarize(requests: List[Request], makespan: float, **extra) -> dict:
+    lat = np.array([r.latency for r in requests])
+    work = float(sum(r.work for r in requests))
+    out = {"requests": len(requests), "makespan_s": makespan, "throughput": work / makespan,
+           "throughput_unit": "grid-point-steps/s", "latency_p50_s": float(np.percentile(lat, 50)),
+           "latency_p99_s": float(np.percentile(lat, 99)), "latency_max_s": float(lat.max())}
+    out.update(extra)
+    return out
+
+
+def klin(dim: int, method: int, h: float, a: float) -> float:
+    """Linear RK4 stability bound used by the reference drivers: h^2/(d sqrt 2 a)
+    for CD and 3/4 of it for 2SHOC (NLSEmagic1D.m:86-94, NLSEmagic2D.m:128-136,
+    NLSEmagic3D.m:154-161)."""
+    return (1.0 if method == 1 else 0.75) * h * h / (dim * np.sqrt(2) * a)
+
+
+def auto_k(dim: int, method: int, h: float, a: float) -> float:
+    """k = 0.8 klin, the reference drivers' default time step."""
+    return 0.8 * klin(dim, method, h, a)
+
+
+def plan_chunks(endtw: float, k: float, numframes: int):
+    """The reference drivers' chunk planner (NLSEmagic1D.m:97-116): the end
+    time is cut to a multiple of k, the steps are split into `numframes`
+    chunks of ceil(steps/numframes) steps, the frame count is reduced when
+    the rounding would add more than a chunk, and the few extra steps are
+    taken.  Returns (chunk_size, numframes, steps, endt)."""
+    endt = endtw - (endtw - np.floor(endtw / k) * k)  # MATLAB mod(endtw, k)
+    steps = int(np.floor(endt / k))
+    if numframes >= steps:
+        numframes = steps
+    chunk_size = int(np.ceil(steps / numframes))
+    extra = chunk_size * numframes - steps
+    if extra > chunk_size:
+        numframes = int(np.ceil(steps / chunk_size))
+        extra = chunk_size * numframes - steps
+    if extra > 0:
+        endt = chunk_size * numframes * k
+        steps = steps + extra
+    return chunk_size, numframes, steps, endt
+
+
+def request_mix(n: int, qps: float, seed: int = 2) -> List[Request]:
+    """The serving workload: Poisson arrivals at `qps`, uniform over
+    {1D, 2D, 3D} x {CD, 2SHOC} x {compensated FP32, FP64}; sizes log-uniform
+    in 1D [4096, 65536] points, 2D [128, 1024]^2, 3D [64, 192]^3; steps
+    log-uniform in [100, 20000].  Per request, from one PCG64 stream seeded
+    with `seed`, in this order: inter-arrival gap, dim, method, precision,
+    size, steps.  s = -1, a = 1, V = 0, Dirichlet boundaries, k = 0.8 klin;
+    h = 0.1 (1D, 2D) or 0.2 (3D); the initial state is a dark soliton (1D),
+    a dark vortex (2D) or a dark vortex ring of radius a quarter of the box
+    (3D)."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    lu = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+    for i in range(n):
+        t += float(rng.exponential(1.0 / qps))
+        dim = int(rng.integers(1, 4))
+        method = int(rng.integers(1, 3))
+        precision = (2, 3)[int(rng.integers(0, 2))]
+        if dim == 1:
+            shape = (int(round(lu(4096, 65536))),)
+        elif dim == 2:
+            side = int(round(lu(128, 1024)))
+            shape = (side, side)
+        else:
+            side = int(round(lu(64, 192)))
+            shape = (side, side, side)
+        steps = int(round(lu(100, 20000)))
+        h = 0.1 if dim < 3 else 0.2
+        out.append(Request(i, t, shape, method, precision, 1, -1.0, 1.0, h * h, auto_k(dim, method, h, 1.0),
+                           steps, _dark_state(shape, h)))
+    return out
+
+
+def _dark_state(shape, h):
+    def make():
+        grids = np.meshgrid(*[h * (np.arange(n) - (n - 1) / 2) for n in shape], indexing="ij")
+        if len(shape) == 1:
+            U = np.tanh(grids[0] / np.sqrt(2)) + 0j
+        elif len(shape) == 2:
+            X, Y = grids
+            U = np.tanh(np.hypot(X, Y)) * np.exp(1j * np.arctan2(Y, X))
+        else:
+            Z, Y, X = grids
+            R = 0.25 * h * shape[-1]
+            rho = np.hypot(X, Y)
+            U = np.tanh(np.hypot(rho - R, Z) / np.sqrt(2)) * np.exp(1j * np.arctan2(Z, rho - R))
+        return U, np.zeros(shape)
+    return make
