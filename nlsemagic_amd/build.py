@@ -44,6 +44,33 @@ def _compile(src: str) -> str:
     return obj
 
 
+MEXDIR = os.path.join(HERE, "mex")
+# MATLAB-visible gateway names -> compile-time identity of nlse_mex_gateway.c
+GATEWAYS = {f"NLSE{d}D_TAKE_STEPS_{m}_CUDA_{p}": [f"-DNLSE_DIM={d}", f"-DNLSE_METHOD={mi}", f"-DNLSE_PREC={pi}"]
+            for d in (1, 2, 3) for m, mi in (("CD", 1), ("2SHOC", 2)) for p, pi in (("D", 2), ("F", 1))}
+GATEWAYS.update({f"NLSEmagic{d}D_Take_Steps_GPU": [f"-DNLSE_DIM={d}", "-DNLSE_V2"] for d in (1, 2, 3)})
+
+
+def build_test_gateways(mex_include: str, extra_src=(), outdir=None) -> dict:
+    """Compile every gateway name against a given mex.h directory (the
+    test-only shim under tests/mex_shim when MATLAB/Octave are absent);
+    returns {name: path of the shared object}."""
+    outdir = outdir or os.path.join(MEXDIR, "_build")
+    os.makedirs(outdir, exist_ok=True)
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    out = {}
+    for name, defs in GATEWAYS.items():
+        so = os.path.join(outdir, name + ".so")
+        cmd = ["gcc", "-std=c99", "-O2", "-fPIC", "-shared", "-Wall", "-Werror", *defs, f"-I{mex_include}",
+               f"-I{inc}", os.path.join(MEXDIR, "nlse_mex_gateway.c"), *extra_src, f"-L{HERE}", "-lnlsemagic_hip",
+               f"-Wl,-rpath,{HERE}", "-o", so]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"gateway {name} failed:\n{r.stderr}")
+        out[name] = so
+    return out
+
+
 def build(verbose: bool = True) -> str:
     """Compile every HIP source and link the shared library; returns its path."""
     os.makedirs(OBJDIR, exist_ok=True)

@@ -1,0 +1,105 @@
+"""The MEX gateway (nlsemagic_amd/mex/nlse_mex_gateway.c) under every
+MATLAB-visible name, driven through the test-only MEX stand-in in
+tests/mex_shim (MATLAB and Octave are absent here).  CPU tests cover the
+argument checks; GPU tests compare each name's result with the oracle."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_D = C.POINTER(C.c_double)
+
+
+@pytest.fixture(scope="module")
+def gateways():
+    from nlsemagic_amd import build as B
+    out = {n: os.path.join(B.MEXDIR, "_build", n + ".so") for n in B.GATEWAYS}
+    if not all(os.path.exists(p) for p in out.values()):
+        shim = os.path.join(ROOT, "tests", "mex_shim")
+        out = B.build_test_gateways(shim, extra_src=[os.path.join(shim, "mex_shim.c")])
+    libs = {}
+    for n, p in out.items():
+        L = C.CDLL(p)
+        L.shim_call.restype = C.c_int
+        L.shim_call.argtypes = [C.c_int, C.POINTER(C.c_size_t), _D, _D, _D, _D, C.c_int, _D, _D, C.c_char_p,
+                                C.c_int]
+        libs[n] = L
+    return libs
+
+
+def call(lib, U, V, *scalars, matlab_dims=None):
+    """U in this project's layout; passed with MATLAB dims (M, N[, L])."""
+    U = np.asarray(U)
+    if matlab_dims is None:
+        matlab_dims = {1: lambda s: (s[0], 1), 2: lambda s: (s[1], s[0]), 3: lambda s: (s[2], s[1], s[0])}[U.ndim](U.shape)
+    dims = (C.c_size_t * len(matlab_dims))(*matlab_dims)
+    ur = np.ascontiguousarray(U.real, dtype=np.float64)
+    ui = np.ascontiguousarray(U.imag, dtype=np.float64) if np.iscomplexobj(U) else None
+    v = np.ascontiguousarray(V, dtype=np.float64)
+    sc = np.asarray(scalars, dtype=np.float64)
+    outr = np.empty(U.shape)
+    outi = np.empty(U.shape)
+    err = C.create_string_buffer(512)
+    rc = lib.shim_call(len(matlab_dims), dims, ur.ctypes.data_as(_D),
+                       ui.ctypes.data_as(_D) if ui is not None else None, v.ctypes.data_as(_D),
+                       sc.ctypes.data_as(_D), len(sc), outr.ctypes.data_as(_D), outi.ctypes.data_as(_D), err, 512)
+    if rc:
+        raise RuntimeError(err.value.decode())
+    out = np.empty(U.shape, complex)
+    out.real, out.imag = outr, outi
+    return out
+
+
+def test_every_name_is_built(gateways):
+    names = set(gateways)
+    for d in (1, 2, 3):
+        assert f"NLSEmagic{d}D_Take_Steps_GPU" in names
+        for m in ("CD", "2SHOC"):
+            for p in ("D", "F"):
+                assert f"NLSE{d}D_TAKE_STEPS_{m}_CUDA_{p}" in names
+
+
+def test_argument_checks_need_no_gpu(gateways):
+    g = gateways["NLSE2D_TAKE_STEPS_2SHOC_CUDA_D"]
+    U = np.ones((6, 5), complex)
+    with pytest.raises(RuntimeError, match="expected 8 arguments"):
+        call(g, U, np.zeros((6, 5)), -1, 1, 0.01, 2, 3)
+    with pytest.raises(RuntimeError, match="expected 10 arguments"):
+        call(gateways["NLSEmagic2D_Take_Steps_GPU"], U, np.zeros((6, 5)), -1, 1, 0.01, 2, 3, 0.001)
+    with pytest.raises(RuntimeError, match="must be a vector"):
+        call(gateways["NLSE1D_TAKE_STEPS_CD_CUDA_D"], U, np.zeros((6, 5)), -1, 1, 0.01, 2, 3, 0.001,
+             matlab_dims=(5, 6))
+    with pytest.raises(RuntimeError, match="precision must be"):
+        call(gateways["NLSEmagic2D_Take_Steps_GPU"], U, np.zeros((6, 5)), -1, 1, 0.01, 2, 3, 0.001, 7, 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("method", [1, 2])
+@pytest.mark.parametrize("prec", [2, 1])
+def test_gateways_match_the_oracle(gateways, oracle, dim, method, prec):
+    shape = {1: (300,), 2: (20, 33), 3: (7, 9, 12)}[dim]
+    rng = np.random.default_rng(dim * 10 + method + prec)
+    U = 1 + 0.2 * rng.uniform(-1, 1, shape) + 0.2j * rng.uniform(-1, 1, shape)
+    V = 0.3 * rng.uniform(-1, 1, shape)
+    args = (-1.0, 0.9, 0.04, 2, 5, 0.004)
+    exp = oracle.take_steps(U, V, *args, method, precision=prec)
+    name = f"NLSE{dim}D_TAKE_STEPS_{'CD' if method == 1 else '2SHOC'}_CUDA_{'D' if prec == 2 else 'F'}"
+    print("CASE", name, flush=True)
+    assert bits_equal(call(gateways[name], U, V, *args), exp)
+    assert bits_equal(call(gateways[f"NLSEmagic{dim}D_Take_Steps_GPU"], U, V, *args, prec, method), exp)
+
+
+@pytest.mark.gpu
+def test_real_input_through_the_gateway(gateways, oracle):
+    rng = np.random.default_rng(1)
+    U = 1 + 0.2 * rng.uniform(-1, 1, (16, 24))
+    V = np.zeros((16, 24))
+    got = call(gateways["NLSE2D_TAKE_STEPS_CD_CUDA_D"], U, V, -1.0, 1.0, 0.01, 1, 4, 0.002)
+    assert bits_equal(got, oracle.take_steps(U + 0j, V, -1.0, 1.0, 0.01, 1, 4, 0.002, 1))
