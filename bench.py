@@ -46,9 +46,16 @@ PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "fused2d_2shoc_f64_1024_traffic
 #  1: four stage launches (56+88+88+72); 2: stages 1-2 read U,V write Y_2,ktot
 #  (56) then stages 3-4 read Y_2,U,V,ktot write U (72); 4: read U,V write U
 DESIGN_BYTES = {1: 304, 2: 128, 4: 40}
-KERNEL_NAME = {1: "fused2d_kernel<double,2SHOC,TI={tile},s,s> (4 launches per RK4 step)",
-               2: "fused2d_kernel<double,2SHOC,TI={tile},1,2> + <..,3,4> (2 launches per RK4 step)",
-               4: "fused2d_kernel<double,2SHOC,TI={tile},1,4> (1 launch per RK4 step)"}
+KERNEL_FAMILY = {0: "fused2d_kernel", 1: "rows2d_kernel", 2: "strip2d_kernel"}
+
+
+def kernel_name(fusion, kind, rows):
+    fam = KERNEL_FAMILY[kind]
+    if fusion == 1:
+        return f"{fam}<double,2SHOC,s,s,rows={rows}> (4 launches per RK4 step)"
+    if fusion == 2:
+        return f"{fam}<double,2SHOC,1,2,rows={rows}> + <..,3,4,..> (2 launches per RK4 step)"
+    return f"{fam}<double,2SHOC,1,4,rows={rows}> (1 launch per RK4 step)"
 
 
 def initial_state():
@@ -315,7 +322,9 @@ def main():
     ap.add_argument("--graph", type=int, default=CHUNK, help="RK4 steps per hipGraph replay (0 = off)")
     ap.add_argument("--fusion", type=int, default=2, choices=(1, 2, 4),
                     help="RK4 stages per kernel launch (results are identical)")
-    ap.add_argument("--tile", type=int, default=8, choices=(8, 16, 32), help="tile height in rows")
+    ap.add_argument("--kernel", type=int, default=0, choices=(0, 1, 2),
+                    help="2D stage kernel: 0 LDS tiles, 1 row-sweep tiles, 2 register strips")
+    ap.add_argument("--tile", type=int, default=8, help="rows per tile (strip segment) of that kernel")
     ap.add_argument("--autotune", action="store_true", help="pick fusion and tile with the built-in tuner")
     ap.add_argument("--qps", type=float, default=4.0, help="serving: Poisson arrival rate")
     ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
@@ -353,10 +362,11 @@ def main():
     sol = nm.Solver((N, M), 2, 2, 2, -1.0, 1.0, H * H, k)
     sol.set_graph(args.graph)
     sol.set_fusion(args.fusion)
-    sol.set_tile(args.tile)
+    sol.set_kernel(args.kernel, args.tile)
     sol.set_state(U, V)
     if args.autotune:
-        args.fusion, args.tile, _ = sol.autotune(5)
+        sol.autotune(5)
+    args.fusion, args.kernel, args.tile = sol.config()
     mass0 = float(np.sum(np.abs(U) ** 2))
 
     for _ in range(args.warmup):
@@ -383,11 +393,11 @@ def main():
     if os.path.exists(PROFILE_TRAFFIC):
         with open(PROFILE_TRAFFIC) as f:
             prof = json.load(f).get(f"fusion{args.fusion}", {})
-        if prof.get("tile") in (None, args.tile):
+        if prof.get("tile") == args.tile and prof.get("kernel", 0) == args.kernel:
             traffic = prof.get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": KERNEL_NAME[args.fusion].format(tile=args.tile),
+                "kernel": kernel_name(args.fusion, args.kernel, args.tile),
                 "launch_us": round(t_launch * 1e6, 2),
                 "algorithmic_bytes_per_point_step": sum(STAGE_BYTES),
                 "design_bytes_per_point_step": design_bytes,
@@ -416,7 +426,7 @@ def main():
         "config": {"workload": "2D 2SHOC RK4 FP64, 1024x1024, MSD, dark vortex + 1e-3 noise (seed 0)",
                    "grid": [N, M], "h": H, "k_over_h2": round(K_RULE, 6), "chunk_size": CHUNK,
                    "rk4_steps_timed": total_steps, "device_ms": dev_ms, "hip_graph_steps": args.graph,
-                   "stages_per_launch": args.fusion, "tile_rows": args.tile,
+                   "stages_per_launch": args.fusion, "kernel_kind": args.kernel, "tile_rows": args.tile,
                    "mass_drift": mass_drift, "finite": finite},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -119,12 +119,24 @@ int nlse_profile_launches(nlse_solver *h, int steps, double *ms_launch, int *nla
 /* 2D: RK4 stages fused per kernel launch, 1, 2 or 4 (default 2; the results
  * are identical for every choice) */
 int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
-/* 2D: tile height of the stage kernels, 8 (default), 16 or 32 rows of 64
- * columns (results identical) */
+/* 2D: the stage kernel family and its rows per tile (results identical for
+ * every choice):
+ *   kind 0  LDS tiles of 64 columns x 8 (default), 16 or 32 rows
+ *   kind 1  row-sweep LDS tiles (one lane per column, DPP neighbours),
+ *           8, 12 or 16 rows
+ *   kind 2  register-marching strips (one wave per strip, no LDS), segments
+ *           of 3..256 rows, for 1 or 2 stages per launch (4 stages per launch
+ *           run on kind 1 with 8 rows) */
+int nlse_set_kernel(nlse_solver *h, int kind, int rows);
+/* nlse_set_kernel(h, 0, tile_rows) */
 int nlse_set_tile(nlse_solver *h, int tile_rows);
-/* 2D: time `steps` steps for every (stages per launch, tile height) pair on a
- * saved copy of the state, keep the fastest, and restore the state; reports
- * the choice and its ms per step (any pointer may be NULL).  1D/3D: no-op. */
+/* the configuration in use: stages per launch, kernel kind, rows (any
+ * pointer may be NULL) */
+int nlse_get_config(const nlse_solver *h, int *fusion, int *kind, int *rows);
+/* 2D: time `steps` steps for every configuration (stages per launch x kernel
+ * kind x rows) on a saved copy of the state, keep the fastest, and restore
+ * the state; reports the stages per launch, the rows and the ms per step
+ * (any pointer may be NULL; nlse_get_config gives the kind).  1D/3D: no-op. */
 int nlse_autotune(nlse_solver *h, int steps, int *best_fusion, int *best_tile, double *best_ms);
 /* replay the chunk loop from a captured hipGraph of `steps_per_graph` steps
  * (0 = plain stream launches, the default) */

@@ -86,6 +86,10 @@ def lib():
     L.nlse_set_tile.argtypes = [C.c_void_p, C.c_int]
     L.nlse_set_fusion.restype = C.c_int
     L.nlse_set_fusion.argtypes = [C.c_void_p, C.c_int]
+    L.nlse_set_kernel.restype = C.c_int
+    L.nlse_set_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.nlse_get_config.restype = C.c_int
+    L.nlse_get_config.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.nlse_sync.restype = C.c_int
     L.nlse_sync.argtypes = [C.c_void_p]
     L.nlse_query.restype = C.c_int
@@ -307,10 +311,11 @@ class Solver:
         return list(out)[: n.value]
 
     def autotune(self, steps=5):
-        """Pick the fastest (stages per launch, tile height) for this problem.
+        """Pick the fastest kernel configuration (stages per launch, kernel
+        kind, rows) for this problem.
 
-        Returns (stages_per_launch, tile_rows, ms_per_step); the state is
-        unchanged."""
+        Returns (stages_per_launch, rows, ms_per_step); config() gives the
+        kind as well.  The state is unchanged."""
         f, t, ms = C.c_int(), C.c_int(), C.c_double()
         _check(self._lib.nlse_autotune(self._h, int(steps), C.byref(f), C.byref(t), C.byref(ms)))
         return f.value, t.value, ms.value
@@ -322,6 +327,18 @@ class Solver:
     def set_fusion(self, stages_per_launch):
         """2D: RK4 stages per kernel launch (1, 2 or 4)."""
         _check(self._lib.nlse_set_fusion(self._h, int(stages_per_launch)))
+
+    def set_kernel(self, kind, rows):
+        """2D stage kernel family and its rows per tile: 0 LDS tiles (8/16/32),
+        1 row-sweep tiles (8/12/16), 2 register strips (3..256; 1-2 stages per
+        launch)."""
+        _check(self._lib.nlse_set_kernel(self._h, int(kind), int(rows)))
+
+    def config(self):
+        """(stages_per_launch, kernel_kind, rows) in use."""
+        f, t, s = C.c_int(), C.c_int(), C.c_int()
+        _check(self._lib.nlse_get_config(self._h, C.byref(f), C.byref(t), C.byref(s)))
+        return f.value, t.value, s.value
 
     def sync(self):
         _check(self._lib.nlse_sync(self._h))

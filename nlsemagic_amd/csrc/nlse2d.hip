@@ -22,6 +22,8 @@
 //
 // Layout: interleaved complex, element (i, j) at i*M + j; i < N is the
 // reference's x index (its second MATLAB dimension), j < M is contiguous.
+#include "nlse2d_rows.h"
+#include "nlse2d_strip.h"
 #include "nlse2d_tile.h"
 
 namespace nlse {
@@ -59,10 +61,100 @@ static hipError_t launch_fused_ti(const StageArgs<T>& a, hipStream_t st)
     return hipGetLastError();
 }
 
-// tile heights offered to the tuner: 8, 16 (default) and 32 rows of 64 columns
+// Register-marching variant (nlse2d_strip.h): 4 independent waves per
+// workgroup, one strip x segment unit each.
+template <typename T, int METHOD, int FIRST, int LAST, bool COMP>
+__global__ __launch_bounds__(256) void strip2d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                      c2<T>* __restrict__ acc, c2<T>* out,
+                                                      const T* __restrict__ V, Coefs<T> c, Strip2 g, int BC,
+                                                      c2<T>* __restrict__ ulo)
+{
+    constexpr int W = StripCfg<METHOD, FIRST, LAST>::W;
+    const int unit = g.unit();
+    if (unit >= g.nunit) return;
+    const int seg = unit / g.nstrip, strip = unit - seg * g.nstrip;
+    strip2d_wave<T, METHOD, FIRST, LAST, COMP>(seg * g.S - g.ioff, g.S, strip * W - g.joff, g.N, g.M, yin, u, acc, out,
+                                         V, c, BC, ulo);
+}
+
+template <typename T, int METHOD, int FIRST, int LAST>
+static hipError_t launch_strip(const StageArgs<T>& a, hipStream_t st)
+{
+    const Strip2 g = make_strip2(a.N, a.M, a.tile, StripCfg<METHOD, FIRST, LAST>::W);
+    if constexpr (std::is_same<T, float>::value && LAST == 4) {  // compensated state: the low part too
+        if (a.ulo) {
+            hipLaunchKernelGGL((strip2d_kernel<T, METHOD, FIRST, LAST, true>), dim3(8 * g.per_xcd), dim3(256), 0,
+                               st, a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((strip2d_kernel<T, METHOD, FIRST, LAST, false>), dim3(8 * g.per_xcd), dim3(256), 0, st, a.yin,
+                       a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+    return hipGetLastError();
+}
+
+// Row-sweep LDS tiles (nlse2d_rows.h): one workgroup per tile of TI rows.
+template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP>
+__global__ __launch_bounds__(256) void rows2d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                     c2<T>* __restrict__ acc, c2<T>* out, const T* __restrict__ V,
+                                                     Coefs<T> c, Grid2 g, int BC, c2<T>* __restrict__ ulo)
+{
+    __shared__ RowsLds<T, METHOD, FIRST, LAST, TI> sm;
+    constexpr int W = RowsCfg<METHOD, FIRST, LAST, TI>::W;
+    const int t = g.map.tile();
+    if (t >= g.map.ntiles) return;
+    const int ti = t / g.tiles_j, tj = t - ti * g.tiles_j;
+    rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP>(sm, ti * TI - g.ioff, tj * W - g.joff, g.N, g.M, yin, u, acc,
+                                                  out, V, c, BC, ulo);
+}
+
+template <typename T, int METHOD, int FIRST, int LAST, int TI>
+static hipError_t launch_rows_ti(const StageArgs<T>& a, hipStream_t st)
+{
+    constexpr int W = RowsCfg<METHOD, FIRST, LAST, TI>::W;
+    Grid2 g;
+    g.N = a.N;
+    g.M = a.M;
+    g.ioff = (a.N % TI == 1) ? 1 : 0;
+    g.joff = (a.M % W == 1) ? 1 : 0;
+    g.tiles_i = (a.N + g.ioff + TI - 1) / TI;
+    g.tiles_j = (a.M + g.joff + W - 1) / W;
+    g.map.ntiles = g.tiles_i * g.tiles_j;
+    g.map.per_xcd = (g.map.ntiles + 7) / 8;
+    if constexpr (std::is_same<T, float>::value && LAST == 4) {
+        if (a.ulo) {
+            hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, true>), dim3(8 * g.map.per_xcd), dim3(256),
+                               0, st, a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, false>), dim3(8 * g.map.per_xcd), dim3(256), 0, st,
+                       a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+    return hipGetLastError();
+}
+
+template <typename T, int METHOD, int FIRST, int LAST>
+static hipError_t launch_rows(const StageArgs<T>& a, hipStream_t st)
+{
+    switch (a.tile) {
+    case 12: return launch_rows_ti<T, METHOD, FIRST, LAST, 12>(a, st);
+    case 16: return launch_rows_ti<T, METHOD, FIRST, LAST, 16>(a, st);
+    default: return launch_rows_ti<T, METHOD, FIRST, LAST, 8>(a, st);
+    }
+}
+
+// the kernel family and its rows per tile (KERNEL_* in nlse_launch.h)
 template <typename T, int METHOD, int FIRST, int LAST>
 static hipError_t launch_fused(const StageArgs<T>& a, hipStream_t st)
 {
+    if (a.kernel == KERNEL_STRIPS) {
+        if constexpr (LAST - FIRST < 2) return launch_strip<T, METHOD, FIRST, LAST>(a, st);
+        StageArgs<T> b = a;  // four stages per launch: row-sweep tiles
+        b.kernel = KERNEL_ROWS;
+        b.tile = 8;
+        return launch_rows<T, METHOD, FIRST, LAST>(b, st);
+    }
+    if (a.kernel == KERNEL_ROWS) return launch_rows<T, METHOD, FIRST, LAST>(a, st);
     switch (a.tile) {
     case 8: return launch_fused_ti<T, METHOD, 8, FIRST, LAST>(a, st);
     case 32: return launch_fused_ti<T, METHOD, 32, FIRST, LAST>(a, st);

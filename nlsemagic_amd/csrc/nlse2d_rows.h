@@ -1,0 +1,263 @@
+// nlse2d_rows.h -- row-sweep LDS tile for fused 2D RK4 stages (device code).
+//
+// A workgroup of four waves advances a tile of TI rows x W = 64 - 2H owned
+// columns through RK4 stages FIRST..LAST (H = R per fused stage):
+//   - lane l is column j0 - H + l, so one row of the extended tile (the owned
+//     block plus H rows/columns of halo) is exactly one wave-wide access:
+//     a coalesced 1 KB (FP64) load from HBM and conflict-free 16 B per lane
+//     in LDS;
+//   - extended row e belongs to wave e mod 4 in every phase, which keeps the
+//     row's Y, D, F, U, V and ktot in that wave's registers; the rows are
+//     loaded all at once at the start (every load in flight together);
+//   - j +- 1 neighbours of a row's own values come from adjacent lanes (DPP
+//     wave shifts), the rows i +- 1 (and the diagonal corners of the 2SHOC
+//     stencil) from the LDS copy of the stage input, and for 2SHOC the
+//     D = Lap_CD(Y)/h^2 rows i +- 1 from an LDS copy of D;
+//   - per stage: D phase, barrier, F phase, barrier, new Y rows written to
+//     LDS, barrier; the last stage writes its rows to HBM.
+// MSD boundary rows take their partner row's D/F from LDS after the barrier
+// that completes it (D: one extra barrier, only in tiles holding row 0 or
+// N-1); MSD boundary columns take the partner lane's values by DPP.  Every
+// owned point gets the reference's operations in its operand order.
+#pragma once
+
+#include "nlse2d_strip.h"
+
+namespace nlse {
+
+template <int METHOD, int FIRST, int LAST, int TI>
+struct RowsCfg {
+    static constexpr int R = (METHOD == 2) ? 2 : 1;
+    static constexpr int NF = LAST - FIRST + 1;
+    static constexpr int H = NF * R;        // halo rows and columns
+    static constexpr int W = 64 - 2 * H;    // owned columns
+    static constexpr int TE = TI + 2 * H;   // extended rows
+    static constexpr int RW = (TE + 3) / 4; // extended rows per wave
+    static constexpr int ND = (METHOD == 2) ? TE - 2 : 1;  // D rows held in LDS (rows 1..TE-2)
+};
+
+template <typename T, int METHOD, int FIRST, int LAST, int TI>
+struct RowsLds {
+    using Cfg = RowsCfg<METHOD, FIRST, LAST, TI>;
+    // one padding element before and after each array: lanes 0 and 63 read
+    // their out-of-row neighbour (a value no owned point uses) in bounds
+    // MSD: F of rows 1 and N-2 for the boundary rows goes to extended rows 0
+    // and TE-1 of y, which no F phase reads
+    c2<T> y[Cfg::TE * 64 + 2];
+    c2<T> d[Cfg::ND * 64 + 2];
+};
+
+// Tile with owned rows [i0, i0 + TI) and columns [j0, j0 + W) of an N x M
+// problem (the block may overhang the grid by one row/column at the start
+// and by any amount at the end).  Buffers as in fused2d_tile.
+template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP>
+__device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>& sm, int i0, int j0, int N, int M,
+                                            const c2<T>* __restrict__ yin, const c2<T>* u, c2<T>* __restrict__ acc,
+                                            c2<T>* out, const T* __restrict__ V, const Coefs<T>& c, int BC,
+                                            c2<T>* __restrict__ ulo)
+{
+    using Cfg = RowsCfg<METHOD, FIRST, LAST, TI>;
+    constexpr int R = Cfg::R, NF = Cfg::NF, H = Cfg::H, W = Cfg::W, TE = Cfg::TE, RW = Cfg::RW;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = j0 - H + lane;
+    const bool jin = j >= 0 && j < M;
+    const bool own = jin && lane >= H && lane < H + W;
+    const int jc = j < 0 ? 0 : (j >= M ? M - 1 : j);
+    const bool colL = (j == 0), colR = (j == M - 1);
+    const int lL = H - j0, lR = M - 1 - j0 + H;
+    const bool colbc = (lL >= 0 && lL < 64) || (lR >= 0 && lR < 64);
+    const int g0 = i0 - H;  // grid row of extended row 0
+    const int eg0 = -g0, eg1 = N - g0;  // extended rows inside the grid: [eg0, eg1)
+    const bool msd = (BC == BC_MSD);
+    const bool top = msd && eg0 >= 0 && eg0 < TE, bot = msd && eg1 - 1 >= 0 && eg1 - 1 < TE;
+    const c2<T> z = mk<T>(0, 0);
+
+    auto sy = [&](int e, int dl) -> c2<T> { return sm.y[1 + e * 64 + lane + dl]; };
+    auto sd = [&](int e, int dl) -> c2<T> { return sm.d[1 + (e - 1) * 64 + lane + dl]; };
+    auto colq = [&](auto v) {
+        const auto p = lane_shift2<DPP_NEXT, T>(v), m = lane_shift2<DPP_PREV, T>(v);
+        return colL ? p : (colR ? m : v);
+    };
+    auto colq1 = [&](T v) {
+        const T p = lane_shift<DPP_NEXT>(v), m = lane_shift<DPP_PREV>(v);
+        return colL ? p : (colR ? m : v);
+    };
+    auto in_grid = [&](int e) { return e >= eg0 && e < eg1; };
+
+    // rows of this wave: e = 4k + w; everything about row e stays in slot k
+    c2<T> Yc[RW], Uc[RW], Ac[RW], Dc[RW], Fc[RW], Lc[RW];
+    T Vc[RW];
+
+    // phase 0: every load of the tile in flight at once
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+        const int e = 4 * k + w;
+        const int g = g0 + e;
+        const int gc = g < 0 ? 0 : (g >= N ? N - 1 : g);
+        const long p = (long)gc * M + jc;
+        const bool row = e < TE;  // every extended row is read (out-of-grid rows give garbage, never used)
+        const bool frow = e >= R && e < TE - R;  // rows the first fused stage updates
+        Yc[k] = row ? yin[p] : z;
+        Vc[k] = row ? V[p] : (T)0;
+        if constexpr (FIRST > 1) {
+            Uc[k] = frow ? u[p] : z;
+            Ac[k] = frow ? acc[p] : z;
+        } else {
+            Ac[k] = z;
+        }
+        if constexpr (COMP && LAST == 4) Lc[k] = (e >= H && e < H + TI) ? ulo[p] : z;
+    }
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+        const int e = 4 * k + w;
+        if constexpr (FIRST == 1) Uc[k] = Yc[k];
+        if (e < TE) sm.y[1 + e * 64 + lane] = Yc[k];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        const int QS = FIRST + q;
+        const int flo = max((q + 1) * R, eg0), fhi = min(TE - (q + 1) * R, eg1);  // F rows
+        const int dlo = max((q + 1) * R - 1, eg0), dhi = min(TE - (q + 1) * R + 1, eg1);  // D rows
+        c2<T> ypart[RW];  // MSD boundary rows: the partner row's Y, read before the Y rows change
+#pragma unroll
+        for (int k = 0; k < RW; ++k) ypart[k] = z;
+        if constexpr (METHOD == 2) {
+#pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int e = 4 * k + w;
+                if (e < dlo || e >= dhi) continue;
+                const int g = g0 + e;
+                const c2<T> yx = Yc[k];
+                const T vx = Vc[k];
+                c2<T> dd = z;
+                if (g > 0 && g < N - 1) {
+                    const c2<T> a = sy(e + 1, 0), b = sy(e - 1, 0);
+                    const c2<T> yp = lane_shift2<DPP_NEXT, T>(yx), ym = lane_shift2<DPP_PREV, T>(yx);
+                    dd = mk<T>((a.x + b.x - 4 * yx.x + yp.x + ym.x) * c.lh2, (a.y + b.y - 4 * yx.y + yp.y + ym.y) * c.lh2);
+                    if (colbc) {
+                        c2<T> db = z;
+                        if (BC == BC_DIRICHLET) db = dir_d<T>(yx, vx, c);
+                        else if (msd) db = msd_d<T>(colq(dd), colq(yx), colq1(vx), yx, vx, c);
+                        if (colL || colR) dd = db;
+                    }
+                } else if (BC == BC_DIRICHLET) {
+                    dd = dir_d<T>(yx, vx, c);
+                }
+                Dc[k] = dd;
+                sm.d[1 + (e - 1) * 64 + lane] = dd;
+            }
+            __syncthreads();
+            if (top || bot) {  // MSD rows 0 / N-1 from rows 1 / N-2 (uniform over the workgroup)
+#pragma unroll
+                for (int k = 0; k < RW; ++k) {
+                    const int e = 4 * k + w;
+                    if (e < dlo || e >= dhi) continue;
+                    const int g = g0 + e;
+                    if (g != 0 && g != N - 1) continue;
+                    const int ep = (g == 0) ? e + 1 : e - 1;  // partner row
+                    const T vp = V[(long)(g == 0 ? 1 : N - 2) * M + jc];
+                    const c2<T> dd = msd_d<T>(colq(sd(ep, 0)), colq(sy(ep, 0)), colq1(vp), Yc[k], Vc[k], c);
+                    Dc[k] = dd;
+                    sm.d[1 + (e - 1) * 64 + lane] = dd;
+                }
+                __syncthreads();
+            }
+        }
+        // F phase: rows into registers; the partner rows of MSD boundary rows
+        // also into LDS
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int e = 4 * k + w;
+            if (e < flo || e >= fhi) continue;
+            const int g = g0 + e;
+            const c2<T> yx = Yc[k];
+            const T vx = Vc[k];
+            c2<T> ff = z;
+            if (g > 0 && g < N - 1) {
+                const T n = nlin<T>(c.s, yx, vx);
+                if constexpr (METHOD == 2) {
+                    const c2<T> dc = Dc[k], da = sd(e + 1, 0), db = sd(e - 1, 0);
+                    const c2<T> dl = lane_shift2<DPP_NEXT, T>(dc), dr = lane_shift2<DPP_PREV, T>(dc);
+                    const c2<T> aa = sy(e + 1, 1), ab = sy(e + 1, -1), ba = sy(e - 1, 1), bb = sy(e - 1, -1);
+                    ff = mk<T>(-c.a * dc.y + c.a1_12 * (da.y + db.y + dl.y + dr.y)
+                                   - c.a1_6h2 * (aa.y + ab.y - 4 * yx.y + ba.y + bb.y) - n * yx.y,
+                               c.a * dc.x - c.a1_12 * (da.x + db.x + dl.x + dr.x)
+                                   + c.a1_6h2 * (aa.x + ab.x - 4 * yx.x + ba.x + bb.x) + n * yx.x);
+                } else {
+                    const c2<T> a = sy(e + 1, 0), b = sy(e - 1, 0);
+                    const c2<T> l = lane_shift2<DPP_NEXT, T>(yx), rt = lane_shift2<DPP_PREV, T>(yx);
+                    ff = mk<T>(-c.ah2 * (a.y - 4 * yx.y + b.y + l.y + rt.y) - n * yx.y,
+                               c.ah2 * (a.x - 4 * yx.x + b.x + l.x + rt.x) + n * yx.x);
+                }
+                if (colbc) {
+                    c2<T> fb = z;
+                    if (msd) fb = msd_f<T>(colq(ff), colq(yx), yx);
+                    else if (BC == BC_LAP0) fb = lap0_f<T>(yx, vx, c.s);
+                    if (colL || colR) ff = fb;
+                }
+                if (top && g == 1) sm.y[1 + lane] = ff;
+                if (bot && g == N - 2) sm.y[1 + (TE - 1) * 64 + lane] = ff;
+            } else if (BC == BC_LAP0) {
+                ff = lap0_f<T>(yx, vx, c.s);
+            } else if (msd) {
+                ypart[k] = sy(g == 0 ? e + 1 : e - 1, 0);
+            }
+            Fc[k] = ff;
+        }
+        __syncthreads();  // every read of this stage's Y and D rows is done
+        if (top || bot) {
+#pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int e = 4 * k + w;
+                if (e < flo || e >= fhi) continue;
+                const int g = g0 + e;
+                if (g == 0) Fc[k] = msd_f<T>(colq(sm.y[1 + lane]), colq(ypart[k]), Yc[k]);
+                else if (g == N - 1) Fc[k] = msd_f<T>(colq(sm.y[1 + (TE - 1) * 64 + lane]), colq(ypart[k]), Yc[k]);
+            }
+        }
+        // RK4 epilogue of this stage's rows
+        const T cs = (QS == 3) ? c.k : c.k2;
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int e = 4 * k + w;
+            if (e < flo || e >= fhi) continue;
+            const c2<T> f = Fc[k];
+            c2<T> a;
+            if (QS == 1) {
+                a = f;
+            } else {
+                const T wt = (QS == 4) ? (T)1.0 : (T)2.0;
+                a = mk<T>(Ac[k].x + wt * f.x, Ac[k].y + wt * f.y);
+            }
+            const c2<T> uo = Uc[k];
+            if (q < NF - 1) {
+                Ac[k] = a;
+                const c2<T> yn = mk<T>(uo.x + cs * f.x, uo.y + cs * f.y);
+                Yc[k] = yn;
+                sm.y[1 + e * 64 + lane] = yn;
+            } else if (own) {  // the last stage's rows are the owned rows
+                const long p = (long)(g0 + e) * M + j;
+                if (QS == 4) {
+                    if constexpr (COMP) {
+                        c2<T> hi = uo, lo = Lc[k];
+                        comp_add<T>(hi.x, lo.x, c.k6 * a.x);
+                        comp_add<T>(hi.y, lo.y, c.k6 * a.y);
+                        ulo[p] = lo;
+                        out[p] = hi;
+                    } else {
+                        out[p] = mk<T>(uo.x + c.k6 * a.x, uo.y + c.k6 * a.y);
+                    }
+                } else {
+                    out[p] = mk<T>(uo.x + cs * f.x, uo.y + cs * f.y);
+                    acc[p] = a;
+                }
+            }
+        }
+        if (q < NF - 1) __syncthreads();  // the next stage's input rows are in LDS
+    }
+}
+
+}  // namespace nlse

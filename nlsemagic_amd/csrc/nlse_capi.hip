@@ -60,7 +60,8 @@ struct nlse_solver {
     int gsteps;               // steps per graph replay (0: plain launches)
     int gcaptured[2];         // steps held by gexec[i]
     int fuse;                 // 2D: RK4 stages per launch (1, 2 or 4)
-    int tile;                 // 2D: tile height (8, 16, 32)
+    int kern;                 // 2D: kernel family (KERNEL_TILES, KERNEL_ROWS, KERNEL_STRIPS)
+    int tile;                 // 2D: rows per tile / strip segment of that family
     void* ubase;              // the buffer U was allocated in (U swaps with Y1 when fuse > 1)
     hipEvent_t ev0, ev1;
 };
@@ -132,6 +133,7 @@ int enqueue_stage_t(nlse_solver* h, const Coefs<T>& c, int stage)
     a.c = c;
     a.N = h->N; a.M = h->M; a.L = h->L;
     a.BC = h->BC; a.method = h->method;
+    a.kernel = h->kern;
     a.tile = h->tile;
     a.stage = stage;
     a.yin = yin[stage - 1];
@@ -165,6 +167,7 @@ int enqueue_launch_t(nlse_solver* h, const Coefs<T>& c, int l)
     a.c = c;
     a.N = h->N; a.M = h->M; a.L = h->L;
     a.BC = h->BC; a.method = h->method;
+    a.kernel = h->kern;
     a.tile = h->tile;
     a.stage = 0;
     a.u = (const C*)h->U;
@@ -407,6 +410,7 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
     }
     h->ubase = h->U;
     h->fuse = (dim == 2) ? 2 : 1;
+    h->kern = KERNEL_TILES;
     h->tile = 8;
     set_params(h, BC, s, a, h2, k);
     // a defined initial state: zero field and potential
@@ -560,12 +564,34 @@ int nlse_set_fusion(nlse_solver* h, int stages_per_launch)
     return NLSE_OK;
 }
 
-int nlse_set_tile(nlse_solver* h, int tile_rows)
+int nlse_set_kernel(nlse_solver* h, int kind, int rows)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");
-    if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32) return fail(NLSE_EINVAL, "tile rows must be 8, 16 or 32");
-    h->tile = tile_rows;
+    if (kind == KERNEL_TILES) {
+        if (rows != 8 && rows != 16 && rows != 32) return fail(NLSE_EINVAL, "LDS tile rows must be 8, 16 or 32");
+    } else if (kind == KERNEL_ROWS) {
+        if (rows != 8 && rows != 12 && rows != 16) return fail(NLSE_EINVAL, "row-sweep tile rows must be 8, 12 or 16");
+    } else if (kind == KERNEL_STRIPS) {
+        // 3 rows at least: with the origin shift every segment then owns two
+        // rows, so an MSD boundary row and its partner row are in one wave
+        if (rows < 3 || rows > 256) return fail(NLSE_EINVAL, "strip segment rows must be 3..256");
+    } else {
+        return fail(NLSE_EINVAL, "kernel kind must be 0 (LDS tiles), 1 (row-sweep tiles) or 2 (register strips)");
+    }
+    h->kern = kind;
+    h->tile = rows;
     drop_graphs(h);
+    return NLSE_OK;
+}
+
+int nlse_set_tile(nlse_solver* h, int tile_rows) { return nlse_set_kernel(h, KERNEL_TILES, tile_rows); }
+
+int nlse_get_config(const nlse_solver* h, int* fusion_out, int* kind, int* rows)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    if (fusion_out) *fusion_out = fusion(h);
+    if (kind) *kind = h->dim == 2 ? h->kern : 0;
+    if (rows) *rows = h->dim == 2 ? h->tile : 0;
     return NLSE_OK;
 }
 
@@ -579,29 +605,44 @@ int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, d
         return NLSE_OK;
     }
     const size_t bytes = elem_bytes(h->prec) * (size_t)h->n;
-    void* saved = nullptr;
-    if (hipMalloc(&saved, bytes) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    void* saved = nullptr;  // U, then the compensated low part when there is one
+    const size_t sbytes = h->Ulo ? 2 * bytes : bytes;
+    if (hipMalloc(&saved, sbytes) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
     HIPCHK(hipMemcpyAsync(saved, h->U, bytes, hipMemcpyDeviceToDevice, h->stream));
-    const int fz[3] = {1, 2, 4}, tl[3] = {8, 16, 32};
+    if (h->Ulo) HIPCHK(hipMemcpyAsync((char*)saved + bytes, h->Ulo, bytes, hipMemcpyDeviceToDevice, h->stream));
+    // candidates: every kernel family and tile height for 1, 2 and 4 stages
+    // per launch (register strips for 1 and 2)
+    struct Cand { int fuse, kern, tile; };
+    const Cand cand[] = {{1, KERNEL_TILES, 8},  {1, KERNEL_TILES, 16}, {1, KERNEL_TILES, 32}, {2, KERNEL_TILES, 8},
+                         {2, KERNEL_TILES, 16}, {2, KERNEL_TILES, 32}, {4, KERNEL_TILES, 8},  {4, KERNEL_TILES, 16},
+                         {4, KERNEL_TILES, 32}, {1, KERNEL_ROWS, 8},   {1, KERNEL_ROWS, 12},  {1, KERNEL_ROWS, 16},
+                         {2, KERNEL_ROWS, 8},   {2, KERNEL_ROWS, 12},  {2, KERNEL_ROWS, 16},  {4, KERNEL_ROWS, 8},
+                         {4, KERNEL_ROWS, 12},  {4, KERNEL_ROWS, 16},  {1, KERNEL_STRIPS, 10}, {1, KERNEL_STRIPS, 16},
+                         {2, KERNEL_STRIPS, 10}, {2, KERNEL_STRIPS, 16}};
     const int gsave = h->gsteps;
     h->gsteps = 0;
-    int bf = h->fuse, bt = h->tile, rc = NLSE_OK;
+    int bf = h->fuse, bk = h->kern, bt = h->tile, rc = NLSE_OK;
     double bms = 1e30;
-    for (int a = 0; a < 3 && rc == NLSE_OK; ++a)
-        for (int b = 0; b < 3 && rc == NLSE_OK; ++b) {
-            h->fuse = fz[a];
-            h->tile = tl[b];
-            rc = enqueue_steps(h, 1);  // warm-up (first launch of this kernel)
-            double ms = 0;
-            if (rc == NLSE_OK) rc = nlse_step_timed(h, steps, &ms);
-            if (rc == NLSE_OK && ms < bms) { bms = ms; bf = fz[a]; bt = tl[b]; }
-            if (hipMemcpyAsync(h->U, saved, bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess && rc == NLSE_OK)
-                rc = fail(NLSE_EHIP, "restore after tuning trial");
-        }
+    for (const Cand& cd : cand) {
+        if (rc != NLSE_OK) break;
+        h->fuse = cd.fuse;
+        h->kern = cd.kern;
+        h->tile = cd.tile;
+        rc = enqueue_steps(h, 1);  // warm-up (first launch of this kernel)
+        double ms = 0;
+        if (rc == NLSE_OK) rc = nlse_step_timed(h, steps, &ms);
+        if (rc == NLSE_OK && ms < bms) { bms = ms; bf = cd.fuse; bk = cd.kern; bt = cd.tile; }
+        if (hipMemcpyAsync(h->U, saved, bytes, hipMemcpyDeviceToDevice, h->stream) != hipSuccess && rc == NLSE_OK)
+            rc = fail(NLSE_EHIP, "restore after tuning trial");
+        if (h->Ulo && hipMemcpyAsync(h->Ulo, (char*)saved + bytes, bytes, hipMemcpyDeviceToDevice, h->stream) !=
+                          hipSuccess && rc == NLSE_OK)
+            rc = fail(NLSE_EHIP, "restore after tuning trial");
+    }
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(saved);
     h->gsteps = gsave;
     h->fuse = bf;
+    h->kern = bk;
     h->tile = bt;
     drop_graphs(h);
     if (rc != NLSE_OK) return rc;

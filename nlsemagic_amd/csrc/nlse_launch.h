@@ -20,8 +20,17 @@ template <typename T> struct StageArgs {
     Coefs<T> c;
     int N, M, L;  // 1D: N; 2D: N x M; 3D: L x N x M (M contiguous)
     int BC, method, stage;
-    int tile;  // 2D: tile height in rows (8, 16 or 32); 0 = default
+    int kernel;  // 2D kernel family: KERNEL_TILES, KERNEL_ROWS or KERNEL_STRIPS
+    int tile;    // 2D: rows per tile (or per strip segment) of that family
 };
+
+// 2D stage kernels (results identical for every choice):
+//   KERNEL_TILES   LDS tiles of 64 columns x 8/16/32 rows, generic 2D thread map
+//   KERNEL_ROWS    row-sweep LDS tiles, lane = column, 8/12/16 rows (nlse2d_rows.h)
+//   KERNEL_STRIPS  register-marching strips, one wave each, 3..256-row segments,
+//                  1 or 2 stages per launch (nlse2d_strip.h); 4 stages per launch
+//                  use KERNEL_ROWS
+enum { KERNEL_TILES = 0, KERNEL_ROWS = 1, KERNEL_STRIPS = 2 };
 
 template <typename T> hipError_t launch_stage1d(const StageArgs<T>& a, hipStream_t st);
 template <typename T> hipError_t launch_stage2d(const StageArgs<T>& a, hipStream_t st);

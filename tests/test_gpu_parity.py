@@ -183,3 +183,99 @@ def test_invalid_arguments_raise(hip):
         hip.take_steps(U, V, -1, 1, 0.01, 2, 1, 0.001, precision=2, method=3)
     with pytest.raises(hip.HipError):
         hip.take_steps(np.ones((2, 8), complex), np.zeros((2, 8)), -1, 1, 0.01, 2, 1, 0.001, precision=2, method=1)
+
+
+def _kernel_cases():
+    """Row-sweep tiles (kind 1) and register strips (kind 2): grids whose sizes
+    hit every decomposition edge (n mod rows == 1, M mod W == 1 for W = 48..62,
+    a last strip or tile of two columns, one strip/segment/tile for the whole
+    grid), both methods, both precisions, every BC, 1/2/4 stages per launch
+    (strips: 1/2), every tile height and 3..32-row strip segments."""
+    out = []
+    seed = 9000
+    shapes = [(3, 3), (4, 5), (3, 100), (100, 3), (9, 57), (17, 61), (33, 63), (5, 113), (7, 121),
+              (40, 58), (25, 62), (12, 64), (65, 130), (34, 200), (10, 169), (13, 49), (21, 50)]
+    strips = (3, 4, 8, 16, 32, 5)
+    for shape in shapes:
+        for method in (1, 2):
+            for prec in (2, 1):
+                for BC in (1, 2, 3, 0):
+                    seed += 1
+                    out.append((shape, method, prec, BC, (1, 2, 4)[seed % 3], 1, (8, 12, 16)[(seed // 3) % 3], seed))
+                    seed += 1
+                    out.append((shape, method, prec, BC, 1 + seed % 2, 2, strips[seed % len(strips)], seed))
+    return out
+
+
+@pytest.mark.parametrize("shape,method,prec,BC,fusion,kind,rows,seed", _kernel_cases())
+def test_kernel_families_2d_match_oracle(hip, oracle, shape, method, prec, BC, fusion, kind, rows, seed):
+    """The row-sweep tiles and the register-marching strips (DPP neighbours)
+    give the reference's bits."""
+    _announce("kernel", shape, method, prec, BC, fusion, kind, rows, seed)
+    rng = np.random.default_rng(seed)
+    U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
+    V = 0.5 * rng.uniform(-1, 1, shape)
+    h2 = 0.04
+    k = 0.1 * h2
+    s = -1.0 if BC != 3 else 0.8
+    exp = oracle.take_steps(U, V, s, 0.9, h2, BC, 3, k, method, precision=prec)
+    sol = hip.Solver(shape, method, prec, BC, s, 0.9, h2, k)
+    sol.set_fusion(fusion)
+    sol.set_kernel(kind, rows)
+    assert sol.config() == (fusion, kind, rows)
+    sol.set_state(U, V)
+    sol.step(3)
+    got = sol.state()
+    sol.close()
+    if prec == 1:
+        got = got.astype(np.complex64).astype(np.complex128)
+    assert bits_equal(got, exp), f"max |diff| {np.max(np.abs(got - exp))}"
+
+
+@pytest.mark.parametrize("kind,fusion,rows", [(1, 1, 8), (1, 2, 8), (1, 2, 16), (1, 4, 12), (2, 1, 16), (2, 2, 10)])
+@pytest.mark.parametrize("method", [1, 2])
+def test_kernel_families_full_size_2d_bitwise(hip, oracle, method, kind, fusion, rows):
+    """1024 x 1024 dark vortex, MSD, a few steps through the row-sweep tiles
+    and the register strips: bit-exact at full size."""
+    n = 1024
+    h = 0.1
+    x = -51.2 + h * np.arange(n)
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    r = np.hypot(X, Y)
+    rng = np.random.default_rng(0)
+    U = np.tanh(r) * np.exp(1j * np.arctan2(Y, X)) + 1e-3 * (rng.uniform(-1, 1, r.shape) + 1j * rng.uniform(-1, 1, r.shape))
+    V = np.zeros(r.shape)
+    k = (0.3 if method == 1 else 0.2121) * h * h
+    _announce("kernel-full2d", method, kind, fusion, rows)
+    sol = hip.Solver((n, n), method, 2, 2, -1.0, 1.0, h * h, k)
+    sol.set_fusion(fusion)
+    sol.set_kernel(kind, rows)
+    sol.set_state(U, V)
+    sol.step(3)
+    got = sol.state()
+    sol.close()
+    exp = oracle.take_steps(U, V, -1.0, 1.0, h * h, 2, 3, k, method)
+    assert bits_equal(got, exp), rel_l2(got, exp)
+
+
+@pytest.mark.parametrize("fusion", [1, 2, 4])
+@pytest.mark.parametrize("method", [1, 2])
+def test_kernel_families_compensated_agree(hip, method, fusion):
+    """Compensated single precision: LDS tiles, row-sweep tiles and register
+    strips give the same bits (hi and lo parts), Dirichlet and MSD."""
+    rng = np.random.default_rng(77 + method)
+    shape = (70, 133)
+    U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
+    V = 0.5 * rng.uniform(-1, 1, shape)
+    for BC in (1, 2):
+        _announce("kernel-comp", method, fusion, BC)
+        res = []
+        for kind, rows in ((0, 8), (1, 12), (2, 8)):
+            sol = hip.Solver(shape, method, 3, BC, -1.0, 0.9, 0.04, 0.004)
+            sol.set_fusion(fusion)
+            sol.set_kernel(kind, rows)
+            sol.set_state(U, V)
+            sol.step(5)
+            res.append(sol.state())
+            sol.close()
+        assert bits_equal(res[0], res[1]) and bits_equal(res[0], res[2])
