@@ -104,8 +104,11 @@ __global__ __launch_bounds__(256) void rows2d_kernel(const c2<T>* __restrict__ y
     const int t = g.map.tile();
     if (t >= g.map.ntiles) return;
     const int ti = t / g.tiles_j, tj = t - ti * g.tiles_j;
-    rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP>(sm, ti * TI - g.ioff, tj * W - g.joff, g.N, g.M, yin, u, acc,
-                                                  out, V, c, BC, ulo);
+    const int i0 = ti * TI - g.ioff, j0 = tj * W - g.joff;
+    if (rows_inner<METHOD, FIRST, LAST, TI>(i0, j0, g.N, g.M))
+        rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true>(sm, i0, j0, g.N, g.M, yin, u, acc, out, V, c, BC, ulo);
+    else
+        rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, false>(sm, i0, j0, g.N, g.M, yin, u, acc, out, V, c, BC, ulo);
 }
 
 template <typename T, int METHOD, int FIRST, int LAST, int TI>

@@ -25,6 +25,13 @@
 
 namespace nlse {
 
+// keeps the compiler from interleaving the rows of a phase (which would hold
+// every row's operands live at once and cut the waves per SIMD); latency is
+// hidden by the other waves instead
+#ifndef ROWS_SCHED_FENCE
+#define ROWS_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
+
 template <int METHOD, int FIRST, int LAST, int TI>
 struct RowsCfg {
     static constexpr int R = (METHOD == 2) ? 2 : 1;
@@ -49,8 +56,17 @@ struct RowsLds {
 
 // Tile with owned rows [i0, i0 + TI) and columns [j0, j0 + W) of an N x M
 // problem (the block may overhang the grid by one row/column at the start
-// and by any amount at the end).  Buffers as in fused2d_tile.
-template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP>
+// and by any amount at the end).  Buffers as in fused2d_tile.  INNER: the
+// extended tile lies inside the grid and holds no boundary row or column
+// (rows_inner()), so every range below is a compile-time constant.
+template <int METHOD, int FIRST, int LAST, int TI>
+__device__ __forceinline__ bool rows_inner(int i0, int j0, int N, int M)
+{
+    using Cfg = RowsCfg<METHOD, FIRST, LAST, TI>;
+    return i0 - Cfg::H >= 0 && i0 - Cfg::H + Cfg::TE <= N && j0 - Cfg::H >= 1 && j0 - Cfg::H + 64 <= M - 1;
+}
+
+template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP, bool INNER>
 __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>& sm, int i0, int j0, int N, int M,
                                             const c2<T>* __restrict__ yin, const c2<T>* u, c2<T>* __restrict__ acc,
                                             c2<T>* out, const T* __restrict__ V, const Coefs<T>& c, int BC,
@@ -60,17 +76,21 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
     constexpr int R = Cfg::R, NF = Cfg::NF, H = Cfg::H, W = Cfg::W, TE = Cfg::TE, RW = Cfg::RW;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __builtin_assume(w >= 0 && w < 4);
     const int j = j0 - H + lane;
-    const bool jin = j >= 0 && j < M;
+    const bool jin = INNER || (j >= 0 && j < M);
     const bool own = jin && lane >= H && lane < H + W;
-    const int jc = j < 0 ? 0 : (j >= M ? M - 1 : j);
+    const int jc = INNER ? j : (j < 0 ? 0 : (j >= M ? M - 1 : j));
     const bool colL = (j == 0), colR = (j == M - 1);
     const int lL = H - j0, lR = M - 1 - j0 + H;
-    const bool colbc = (lL >= 0 && lL < 64) || (lR >= 0 && lR < 64);
+    const bool colbc = !INNER && ((lL >= 0 && lL < 64) || (lR >= 0 && lR < 64));
     const int g0 = i0 - H;  // grid row of extended row 0
-    const int eg0 = -g0, eg1 = N - g0;  // extended rows inside the grid: [eg0, eg1)
+    // extended rows inside the grid: [eg0, eg1)
+    const int eg0 = INNER ? 0 : -g0, eg1 = INNER ? TE : N - g0;
     const bool msd = (BC == BC_MSD);
-    const bool top = msd && eg0 >= 0 && eg0 < TE, bot = msd && eg1 - 1 >= 0 && eg1 - 1 < TE;
+    const bool top = !INNER && msd && eg0 >= 0 && eg0 < TE, bot = !INNER && msd && eg1 - 1 >= 0 && eg1 - 1 < TE;
+    // a D/F row at grid row g is interior (the reference's full stencil)
+    auto interior = [&](int g) { return INNER || (g > 0 && g < N - 1); };
     const c2<T> z = mk<T>(0, 0);
 
     auto sy = [&](int e, int dl) -> c2<T> { return sm.y[1 + e * 64 + lane + dl]; };
@@ -83,7 +103,6 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
         const T p = lane_shift<DPP_NEXT>(v), m = lane_shift<DPP_PREV>(v);
         return colL ? p : (colR ? m : v);
     };
-    auto in_grid = [&](int e) { return e >= eg0 && e < eg1; };
 
     // rows of this wave: e = 4k + w; everything about row e stays in slot k
     c2<T> Yc[RW], Uc[RW], Ac[RW], Dc[RW], Fc[RW], Lc[RW];
@@ -94,7 +113,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
     for (int k = 0; k < RW; ++k) {
         const int e = 4 * k + w;
         const int g = g0 + e;
-        const int gc = g < 0 ? 0 : (g >= N ? N - 1 : g);
+        const int gc = INNER ? g : (g < 0 ? 0 : (g >= N ? N - 1 : g));
         const long p = (long)gc * M + jc;
         const bool row = e < TE;  // every extended row is read (out-of-grid rows give garbage, never used)
         const bool frow = e >= R && e < TE - R;  // rows the first fused stage updates
@@ -119,8 +138,11 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
         const int QS = FIRST + q;
-        const int flo = max((q + 1) * R, eg0), fhi = min(TE - (q + 1) * R, eg1);  // F rows
-        const int dlo = max((q + 1) * R - 1, eg0), dhi = min(TE - (q + 1) * R + 1, eg1);  // D rows
+        // F rows and D rows of this stage
+        const int flo = INNER ? (q + 1) * R : max((q + 1) * R, eg0);
+        const int fhi = INNER ? TE - (q + 1) * R : min(TE - (q + 1) * R, eg1);
+        const int dlo = INNER ? (q + 1) * R - 1 : max((q + 1) * R - 1, eg0);
+        const int dhi = INNER ? TE - (q + 1) * R + 1 : min(TE - (q + 1) * R + 1, eg1);
         c2<T> ypart[RW];  // MSD boundary rows: the partner row's Y, read before the Y rows change
 #pragma unroll
         for (int k = 0; k < RW; ++k) ypart[k] = z;
@@ -133,7 +155,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 const c2<T> yx = Yc[k];
                 const T vx = Vc[k];
                 c2<T> dd = z;
-                if (g > 0 && g < N - 1) {
+                if (interior(g)) {
                     const c2<T> a = sy(e + 1, 0), b = sy(e - 1, 0);
                     const c2<T> yp = lane_shift2<DPP_NEXT, T>(yx), ym = lane_shift2<DPP_PREV, T>(yx);
                     dd = mk<T>((a.x + b.x - 4 * yx.x + yp.x + ym.x) * c.lh2, (a.y + b.y - 4 * yx.y + yp.y + ym.y) * c.lh2);
@@ -148,6 +170,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 }
                 Dc[k] = dd;
                 sm.d[1 + (e - 1) * 64 + lane] = dd;
+                ROWS_SCHED_FENCE();
             }
             __syncthreads();
             if (top || bot) {  // MSD rows 0 / N-1 from rows 1 / N-2 (uniform over the workgroup)
@@ -176,7 +199,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
             const c2<T> yx = Yc[k];
             const T vx = Vc[k];
             c2<T> ff = z;
-            if (g > 0 && g < N - 1) {
+            if (interior(g)) {
                 const T n = nlin<T>(c.s, yx, vx);
                 if constexpr (METHOD == 2) {
                     const c2<T> dc = Dc[k], da = sd(e + 1, 0), db = sd(e - 1, 0);
@@ -206,6 +229,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 ypart[k] = sy(g == 0 ? e + 1 : e - 1, 0);
             }
             Fc[k] = ff;
+            ROWS_SCHED_FENCE();
         }
         __syncthreads();  // every read of this stage's Y and D rows is done
         if (top || bot) {
