@@ -11,11 +11,13 @@ rule k = 0.8 * 0.75 * h^2 / (2 sqrt 2) = 0.2121 h^2 (NLSEmagic2D.m:131-136).
 One bench "step" = one chunk of 50 RK4 steps on the device-resident state
 (U, V already in HBM when the timed region starts).  `value` is grid-point
 RK4 steps per second.  Also reported:
-  roofline      per-kernel HIP-event timing vs the algorithmic HBM bytes of
-                one fused kernel per RK4 stage (56/88/88/72 = 304 B per
-                point-step); the default kernel fuses all four stages in LDS
-                and moves 40 B per point-step, so `frac` can exceed 1 and
-                `design_frac` gives the fraction on its own 40 B
+  roofline      mean launch time of the stage kernel (HIP events around the
+                timed hipGraph replays) vs the algorithmic HBM bytes of one
+                kernel per RK4 stage (56/88/88/72 = 304 B per point-step);
+                the default kernel (row-sweep LDS tiles of 8 rows) fuses two
+                stages per launch and moves 128 B per point-step, so `frac`
+                can exceed 1 and `design_frac` gives the fraction on its own
+                128 B
   cpu_baseline  the reference's own CPU integrator (oracle/_ref, built from
                 src/NLSEmagic2D/NLSEmagic2D_CPU.c), single thread, on a
                 bounded sample of the same workload
@@ -135,7 +137,8 @@ def run_batched(args):
         states = [batched_state(p) for p in probs]
         for i, (U, V) in enumerate(states):
             B.set_state(i, U, V)
-        f, t, g, ms_tune = B.autotune(3)
+        B.autotune(3)
+        f, kind, t, g = B.config()
         B.configure(f, t, g, chunk)
         B.step(chunk)  # warm-up: captures the chunk graph
         B.sync()
@@ -147,7 +150,7 @@ def run_batched(args):
         diag = B.diagnostics()
         finals[prec] = [B.state(i) for i in range(len(probs))] if prec == 3 or True else None
         m0 = np.array([np.sum(np.abs(U) ** 2) for U, _ in states])
-        res[prec] = dict(rate=npts * chunk * nchunks / (sum(lat) * 1e-3), chunk_ms=lat, fusion=f, tile=t, grid=g,
+        res[prec] = dict(rate=npts * chunk * nchunks / (sum(lat) * 1e-3), chunk_ms=lat, fusion=f, kind=kind, tile=t, grid=g,
                          mass=diag[:, 0], mass_drift=diag[:, 0] / m0 - 1, nonfinite=int(diag[:, 2].sum()))
         B.close()
     # FP32c vs FP64 per problem (the FP64 batch is the FP64 reference at full size)
@@ -177,7 +180,8 @@ def run_batched(args):
                    "nonfinite": res[2]["nonfinite"] + res[3]["nonfinite"],
                    "chunk_ms_p50": {k: pct(v["chunk_ms"], 50) for k, v in (("fp64", res[2]), ("fp32c", res[3]))},
                    "chunk_ms_p99": {k: pct(v["chunk_ms"], 99) for k, v in (("fp64", res[2]), ("fp32c", res[3]))},
-                   "tuned": {k: (v["fusion"], v["tile"], v["grid"]) for k, v in (("fp64", res[2]), ("fp32c", res[3]))},
+                   "tuned": {k: {"stages_per_launch": v["fusion"], "kernel_kind": v["kind"], "rows": v["tile"],
+                                 "grid": v["grid"]} for k, v in (("fp64", res[2]), ("fp32c", res[3]))},
                    "fp64_pinned_to_oracle": f"{pinned}/3 smallest problems bit-identical after {chunk * nchunks} steps"},
     }
     print(json.dumps(line))
@@ -322,7 +326,7 @@ def main():
     ap.add_argument("--graph", type=int, default=CHUNK, help="RK4 steps per hipGraph replay (0 = off)")
     ap.add_argument("--fusion", type=int, default=2, choices=(1, 2, 4),
                     help="RK4 stages per kernel launch (results are identical)")
-    ap.add_argument("--kernel", type=int, default=0, choices=(0, 1, 2),
+    ap.add_argument("--kernel", type=int, default=1, choices=(0, 1, 2),
                     help="2D stage kernel: 0 LDS tiles, 1 row-sweep tiles, 2 register strips")
     ap.add_argument("--tile", type=int, default=8, help="rows per tile (strip segment) of that kernel")
     ap.add_argument("--autotune", action="store_true", help="pick fusion and tile with the built-in tuner")

@@ -121,9 +121,9 @@ int nlse_profile_launches(nlse_solver *h, int steps, double *ms_launch, int *nla
 int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
 /* 2D: the stage kernel family and its rows per tile (results identical for
  * every choice):
- *   kind 0  LDS tiles of 64 columns x 8 (default), 16 or 32 rows
+ *   kind 0  LDS tiles of 64 columns x 8, 16 or 32 rows
  *   kind 1  row-sweep LDS tiles (one lane per column, DPP neighbours),
- *           8, 12 or 16 rows
+ *           8 (default), 12 or 16 rows
  *   kind 2  register-marching strips (one wave per strip, no LDS), segments
  *           of 3..256 rows, for 1 or 2 stages per launch (4 stages per launch
  *           run on kind 1 with 8 rows) */
@@ -164,12 +164,19 @@ int nlse_batch_step_timed(nlse_batch *b, int steps, double *ms);
 int nlse_batch_sync(nlse_batch *b);
 /* per-problem diagnostics, out[3p..3p+2] as for nlse_diagnostics */
 int nlse_batch_diagnostics(nlse_batch *b, double *out);
-/* stages per launch (1, 2, 4), tile rows (8, 16, 32), persistent grid
- * (multiple of 8 workgroups) and hipGraph steps per replay (0 = off) */
+/* stage kernel family: 0 LDS tiles (rows 8, 16, 32), 1 row-sweep tiles
+ * (rows 8 (default), 12, 16); see nlse_set_kernel */
+int nlse_batch_set_kernel(nlse_batch *b, int kind, int rows);
+/* stages per launch, kernel kind, rows per tile, persistent grid */
+int nlse_batch_get_config(const nlse_batch *b, int *fusion, int *kind, int *rows, int *grid);
+/* stages per launch (1, 2, 4), rows per tile of the current kernel family,
+ * persistent grid (multiple of 8 workgroups) and hipGraph steps per replay
+ * (0 = off) */
 int nlse_batch_configure(nlse_batch *b, int stages_per_launch, int tile_rows, int grid,
                          int steps_per_graph);
-/* time `steps` steps for each (stages per launch, tile rows, grid) on a saved
- * copy of the state, keep the fastest, restore the state */
+/* time `steps` steps for each (kernel family, stages per launch, tile rows,
+ * grid) on a saved copy of the state, keep the fastest, restore the state
+ * (nlse_batch_get_config gives the family) */
 int nlse_batch_autotune(nlse_batch *b, int steps, int *best_fusion, int *best_tile, int *best_grid,
                         double *best_ms);
 

@@ -116,6 +116,10 @@ def lib():
     L.nlse_batch_sync.argtypes = [C.c_void_p]
     L.nlse_batch_configure.restype = C.c_int
     L.nlse_batch_configure.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+    L.nlse_batch_set_kernel.restype = C.c_int
+    L.nlse_batch_set_kernel.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.nlse_batch_get_config.restype = C.c_int
+    L.nlse_batch_get_config.argtypes = [C.c_void_p, _I, _I, _I, _I]
     L.nlse_batch_autotune.restype = C.c_int
     L.nlse_batch_autotune.argtypes = [C.c_void_p, C.c_int, _I, _I, _I, C.POINTER(C.c_double)]
     L.nlse_diagnostics.restype = C.c_int
@@ -428,8 +432,21 @@ class Batch:
         _check(self._lib.nlse_batch_configure(self._h, int(stages_per_launch), int(tile_rows), int(grid),
                                               int(steps_per_graph)))
 
+    def set_kernel(self, kind, rows):
+        """Stage kernel family: 0 LDS tiles (8/16/32 rows), 1 row-sweep tiles
+        (8/12/16 rows)."""
+        _check(self._lib.nlse_batch_set_kernel(self._h, int(kind), int(rows)))
+
+    def config(self):
+        """(stages_per_launch, kernel_kind, rows, grid) in use."""
+        f, k, t, g = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(self._lib.nlse_batch_get_config(self._h, C.byref(f), C.byref(k), C.byref(t), C.byref(g)))
+        return f.value, k.value, t.value, g.value
+
     def autotune(self, steps=3):
-        """Pick (stages per launch, tile rows, grid); returns them and ms/step."""
+        """Pick (kernel family, stages per launch, tile rows, grid); returns
+        (stages per launch, tile rows, grid, ms/step); config() gives the
+        family."""
         f, t, g, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
         _check(self._lib.nlse_batch_autotune(self._h, int(steps), C.byref(f), C.byref(t), C.byref(g),
                                              C.byref(ms)))

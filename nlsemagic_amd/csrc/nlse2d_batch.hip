@@ -1,8 +1,9 @@
 // nlse2d_batch.hip -- many independent 2D problems (uneven sizes, own s, a,
 // h, k and V each) advanced by ONE persistent launch per fused group of RK4
 // stages.  Each workgroup walks the concatenated tile list of all problems;
-// the tile code is the single-problem one (nlse2d_tile.h), so every problem's
-// result is bit-identical to running it alone.
+// the tile code is the single-problem one (nlse2d_tile.h, nlse2d_rows.h), so
+// every problem's result is bit-identical to running it alone.
+#include "nlse2d_rows.h"
 #include "nlse2d_tile.h"
 
 namespace nlse {
@@ -34,6 +35,52 @@ __global__ __launch_bounds__(256) void batch2d_kernel(const c2<T>* __restrict__ 
     }
 }
 
+// the same walk over row-sweep tiles (tile width 64 - 2H per problem)
+template <typename T, int METHOD, int TI, int FIRST, int LAST, bool COMP>
+__global__ __launch_bounds__(256) void batch2d_rows_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                           c2<T>* __restrict__ acc, c2<T>* out,
+                                                           const T* __restrict__ V,
+                                                           const BatchProb2<T>* __restrict__ probs,
+                                                           const int* __restrict__ tile_prob, int ntiles,
+                                                           c2<T>* __restrict__ ulo)
+{
+    __shared__ RowsLds<T, METHOD, FIRST, LAST, TI> sm;
+    constexpr int W = RowsCfg<METHOD, FIRST, LAST, TI>::W;
+    const int xcd = blockIdx.x & 7, nper = gridDim.x >> 3;
+    const int per = (ntiles + 7) >> 3;
+    for (int i = blockIdx.x >> 3; i < per; i += nper) {
+        const int t = xcd * per + i;
+        if (t >= ntiles) break;
+        const BatchProb2<T>& P = probs[tile_prob[t]];
+        const int lt = t - P.tile0;
+        const int ti = lt / P.tiles_j, tj = lt - ti * P.tiles_j;
+        const int i0 = ti * TI - P.ioff, j0 = tj * W - P.joff;
+        c2<T>* lo = COMP ? ulo + P.off : nullptr;
+        if (rows_inner<METHOD, FIRST, LAST, TI>(i0, j0, P.N, P.M))
+            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true>(sm, i0, j0, P.N, P.M, yin + P.off, u + P.off,
+                                                                acc + P.off, out + P.off, V + P.off, P.c, P.BC, lo);
+        else
+            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, false>(sm, i0, j0, P.N, P.M, yin + P.off, u + P.off,
+                                                                 acc + P.off, out + P.off, V + P.off, P.c, P.BC, lo);
+        __syncthreads();  // the next tile reuses the LDS
+    }
+}
+
+template <typename T, int METHOD, int TI, int FIRST, int LAST>
+static hipError_t launch_b_rows(const BatchArgs<T>& a, hipStream_t st)
+{
+    if constexpr (std::is_same<T, float>::value && LAST == 4) {
+        if (a.ulo) {
+            hipLaunchKernelGGL((batch2d_rows_kernel<T, METHOD, TI, FIRST, LAST, true>), dim3(a.grid), dim3(256), 0,
+                               st, a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.ulo);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((batch2d_rows_kernel<T, METHOD, TI, FIRST, LAST, false>), dim3(a.grid), dim3(256), 0, st,
+                       a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.ulo);
+    return hipGetLastError();
+}
+
 template <typename T, int METHOD, int TI, int FIRST, int LAST>
 static hipError_t launch_b(const BatchArgs<T>& a, hipStream_t st)
 {
@@ -45,6 +92,13 @@ static hipError_t launch_b(const BatchArgs<T>& a, hipStream_t st)
 template <typename T, int METHOD, int FIRST, int LAST>
 static hipError_t launch_b_t(const BatchArgs<T>& a, hipStream_t st)
 {
+    if (a.kernel == KERNEL_ROWS) {
+        switch (a.tile) {
+        case 12: return launch_b_rows<T, METHOD, 12, FIRST, LAST>(a, st);
+        case 16: return launch_b_rows<T, METHOD, 16, FIRST, LAST>(a, st);
+        default: return launch_b_rows<T, METHOD, 8, FIRST, LAST>(a, st);
+        }
+    }
     switch (a.tile) {
     case 8: return launch_b<T, METHOD, 8, FIRST, LAST>(a, st);
     case 32: return launch_b<T, METHOD, 32, FIRST, LAST>(a, st);

@@ -22,7 +22,7 @@ struct nlse_batch {
     void* probs;     // device BatchProb2<T>[nprob]
     int* tile_prob;  // device: problem of each tile
     int ntiles;
-    int fuse, tile, grid;
+    int fuse, kern, tile, grid;
     hipStream_t stream;
     hipGraphExec_t gexec[2];
     int gsteps, gcaptured[2];
@@ -44,11 +44,19 @@ void drop_graphs(nlse_batch* b)
     }
 }
 
-// (re)build the tile list and per-problem descriptors for the current tile height
+// tile width of the current kernel family: 64 columns, or for row-sweep tiles
+// 64 minus the halo of the fused stages on both sides
+int tile_width(const nlse_batch* b)
+{
+    if (b->kern != KERNEL_ROWS) return 64;
+    return 64 - 2 * b->fuse * (b->method == 2 ? 2 : 1);
+}
+
+// (re)build the tile list and per-problem descriptors for the current kernel
 template <typename T>
 int build_tiles_t(nlse_batch* b)
 {
-    const int TI = b->tile;
+    const int TI = b->tile, TW = tile_width(b);
     std::vector<BatchProb2<T>> P(b->nprob);
     std::vector<int> tp;
     int t0 = 0;
@@ -58,9 +66,9 @@ int build_tiles_t(nlse_batch* b)
         q.M = b->M[p];
         q.BC = b->BC;
         q.ioff = (q.N % TI == 1) ? 1 : 0;
-        q.joff = (q.M % 64 == 1) ? 1 : 0;
+        q.joff = (q.M % TW == 1) ? 1 : 0;
         q.tiles_i = (q.N + q.ioff + TI - 1) / TI;
-        q.tiles_j = (q.M + q.joff + 63) / 64;
+        q.tiles_j = (q.M + q.joff + TW - 1) / TW;
         q.tile0 = t0;
         q.off = b->off[p];
         coefs2d(q.c, (T)b->s[p], (T)b->a[p], (T)b->h2[p], (T)b->k[p]);
@@ -102,6 +110,7 @@ int launch_t(nlse_batch* b, int first, int last, const void* yin, void* out)
     a.last = last;
     a.tile = b->tile;
     a.grid = b->grid;
+    a.kernel = b->kern;
     hipError_t e = launch_batch2d<T>(a, b->stream);
     if (e != hipSuccess) return fail(NLSE_EHIP, std::string("batch launch: ") + hipGetErrorString(e));
     return NLSE_OK;
@@ -214,6 +223,7 @@ int nlse_batch_create(nlse_batch** out, int nprob, const int* N, const int* M, c
     }
     b->ntot = ntot;
     b->fuse = 2;
+    b->kern = KERNEL_ROWS;
     b->tile = 8;
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
@@ -351,21 +361,49 @@ int nlse_batch_sync(nlse_batch* b)
     return NLSE_OK;
 }
 
+int nlse_batch_set_kernel(nlse_batch* b, int kind, int rows)
+{
+    if (!b) return fail(NLSE_EINVAL, "null batch");
+    if (kind == KERNEL_TILES) {
+        if (rows != 8 && rows != 16 && rows != 32) return fail(NLSE_EINVAL, "LDS tile rows must be 8, 16 or 32");
+    } else if (kind == KERNEL_ROWS) {
+        if (rows != 8 && rows != 12 && rows != 16) return fail(NLSE_EINVAL, "row-sweep tile rows must be 8, 12 or 16");
+    } else {
+        return fail(NLSE_EINVAL, "batch kernel kind must be 0 (LDS tiles) or 1 (row-sweep tiles)");
+    }
+    b->kern = kind;
+    b->tile = rows;
+    return build_tiles(b);
+}
+
+int nlse_batch_get_config(const nlse_batch* b, int* fusion, int* kind, int* rows, int* grid)
+{
+    if (!b) return fail(NLSE_EINVAL, "null batch");
+    if (fusion) *fusion = b->fuse;
+    if (kind) *kind = b->kern;
+    if (rows) *rows = b->tile;
+    if (grid) *grid = b->grid;
+    return NLSE_OK;
+}
+
 int nlse_batch_configure(nlse_batch* b, int stages_per_launch, int tile_rows, int grid, int steps_per_graph)
 {
     if (!b) return fail(NLSE_EINVAL, "null batch");
     if (stages_per_launch != 1 && stages_per_launch != 2 && stages_per_launch != 4)
         return fail(NLSE_EINVAL, "stages per launch must be 1, 2 or 4");
-    if (tile_rows != 8 && tile_rows != 16 && tile_rows != 32) return fail(NLSE_EINVAL, "tile rows must be 8, 16 or 32");
+    if (b->kern == KERNEL_TILES && tile_rows != 8 && tile_rows != 16 && tile_rows != 32)
+        return fail(NLSE_EINVAL, "LDS tile rows must be 8, 16 or 32");
+    if (b->kern == KERNEL_ROWS && tile_rows != 8 && tile_rows != 12 && tile_rows != 16)
+        return fail(NLSE_EINVAL, "row-sweep tile rows must be 8, 12 or 16");
     if (grid < 8 || grid % 8 != 0) return fail(NLSE_EINVAL, "grid must be a positive multiple of 8");
     if (steps_per_graph < 0) return fail(NLSE_EINVAL, "steps per graph must be >= 0");
-    b->fuse = stages_per_launch;
     b->grid = grid;
     b->gsteps = steps_per_graph;
     drop_graphs(b);
-    if (tile_rows != b->tile) {
+    if (tile_rows != b->tile || stages_per_launch != b->fuse) {
+        b->fuse = stages_per_launch;
         b->tile = tile_rows;
-        return build_tiles(b);
+        return build_tiles(b);  // the row-sweep tile width depends on the stages per launch
     }
     return NLSE_OK;
 }
@@ -384,32 +422,38 @@ int nlse_batch_autotune(nlse_batch* b, int steps, int* best_fusion, int* best_ti
     if (saved_lo) HIPCHK(hipMemcpyAsync(saved_lo, b->Ulo, eb, hipMemcpyDeviceToDevice, b->stream));
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
-    const int fz[3] = {1, 2, 4}, tl[3] = {8, 16, 32}, occ[3] = {2, 4, 8};
+    const int fz[3] = {1, 2, 4}, occ[3] = {2, 4, 8};
+    const int tl[2][3] = {{8, 16, 32}, {8, 12, 16}};  // rows per tile of each kernel family
     const int gsave = b->gsteps;
-    int bf = b->fuse, bt = b->tile, bg = b->grid, rc = NLSE_OK;
+    int bf = b->fuse, bk = b->kern, bt = b->tile, bg = b->grid, rc = NLSE_OK;
     double bms = 1e30;
-    for (int x = 0; x < 3 && rc == NLSE_OK; ++x)
-        for (int y = 0; y < 3 && rc == NLSE_OK; ++y)
-            for (int z = 0; z < 3 && rc == NLSE_OK; ++z) {
-                rc = nlse_batch_configure(b, fz[x], tl[y], 8 * ((cus * occ[z] + 7) / 8), 0);
-                if (rc == NLSE_OK) rc = enqueue_steps(b, 1);
-                double ms = 0;
-                if (rc == NLSE_OK) rc = nlse_batch_step_timed(b, steps, &ms);
-                if (rc == NLSE_OK && ms < bms) {
-                    bms = ms;
-                    bf = fz[x];
-                    bt = tl[y];
-                    bg = b->grid;
+    for (int kind = 0; kind < 2 && rc == NLSE_OK; ++kind)
+        for (int x = 0; x < 3 && rc == NLSE_OK; ++x)
+            for (int y = 0; y < 3 && rc == NLSE_OK; ++y)
+                for (int z = 0; z < 3 && rc == NLSE_OK; ++z) {
+                    rc = nlse_batch_set_kernel(b, kind, tl[kind][y]);
+                    if (rc == NLSE_OK) rc = nlse_batch_configure(b, fz[x], tl[kind][y], 8 * ((cus * occ[z] + 7) / 8), 0);
+                    if (rc == NLSE_OK) rc = enqueue_steps(b, 1);
+                    double ms = 0;
+                    if (rc == NLSE_OK) rc = nlse_batch_step_timed(b, steps, &ms);
+                    if (rc == NLSE_OK && ms < bms) {
+                        bms = ms;
+                        bf = fz[x];
+                        bk = kind;
+                        bt = tl[kind][y];
+                        bg = b->grid;
+                    }
+                    if (hipMemcpyAsync(b->U, saved, eb, hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
+                        (saved_lo &&
+                         hipMemcpyAsync(b->Ulo, saved_lo, eb, hipMemcpyDeviceToDevice, b->stream) != hipSuccess))
+                        if (rc == NLSE_OK) rc = fail(NLSE_EHIP, "restore after tuning trial");
                 }
-                if (hipMemcpyAsync(b->U, saved, eb, hipMemcpyDeviceToDevice, b->stream) != hipSuccess ||
-                    (saved_lo && hipMemcpyAsync(b->Ulo, saved_lo, eb, hipMemcpyDeviceToDevice, b->stream) != hipSuccess))
-                    if (rc == NLSE_OK) rc = fail(NLSE_EHIP, "restore after tuning trial");
-            }
     (void)hipStreamSynchronize(b->stream);
     (void)hipFree(saved);
     if (saved_lo) (void)hipFree(saved_lo);
     if (rc != NLSE_OK) return rc;
-    rc = nlse_batch_configure(b, bf, bt, bg, gsave);
+    rc = nlse_batch_set_kernel(b, bk, bt);
+    if (rc == NLSE_OK) rc = nlse_batch_configure(b, bf, bt, bg, gsave);
     if (best_fusion) *best_fusion = bf;
     if (best_tile) *best_tile = bt;
     if (best_grid) *best_grid = bg;

@@ -65,6 +65,7 @@ template <typename T> struct BatchArgs {
     int ntiles;
     c2<T>* ulo;
     int method, first, last, tile, grid;
+    int kernel;  // KERNEL_TILES or KERNEL_ROWS
 };
 template <typename T> hipError_t launch_batch2d(const BatchArgs<T>& a, hipStream_t st);
 

@@ -67,14 +67,18 @@ def _batch_inputs(seed, method):
 
 
 @pytest.mark.parametrize("method", [1, 2])
-@pytest.mark.parametrize("fusion,tile,grid,graph", [(1, 8, 64, 0), (2, 8, 1024, 0), (2, 16, 256, 4),
-                                                    (4, 32, 8, 0), (4, 16, 2048, 3)])
-def test_fp64_batch_is_bitwise_each_problem_alone(hip, oracle, method, fusion, tile, grid, graph):
-    _announce("batch64", method, fusion, tile, grid, graph)
+@pytest.mark.parametrize("kind,fusion,tile,grid,graph", [(0, 1, 8, 64, 0), (0, 2, 8, 1024, 0), (0, 2, 16, 256, 4),
+                                                         (0, 4, 32, 8, 0), (0, 4, 16, 2048, 3), (1, 1, 8, 64, 0),
+                                                         (1, 2, 8, 1024, 0), (1, 2, 12, 256, 4), (1, 4, 16, 8, 0),
+                                                         (1, 4, 8, 2048, 3)])
+def test_fp64_batch_is_bitwise_each_problem_alone(hip, oracle, method, kind, fusion, tile, grid, graph):
+    _announce("batch64", method, kind, fusion, tile, grid, graph)
     probs = _batch_inputs(10 + method, method)
     B = hip.Batch([p["shape"] for p in probs], [p["s"] for p in probs], [p["a"] for p in probs],
                   [p["h2"] for p in probs], [p["k"] for p in probs], method=method, precision=2, BC=1)
+    B.set_kernel(kind, tile)
     B.configure(fusion, tile, grid, graph)
+    assert B.config() == (fusion, kind, tile, grid)
     for i, p in enumerate(probs):
         B.set_state(i, p["U"], p["V"])
     B.step(5)
@@ -120,12 +124,13 @@ def test_compensated_single_precision_accuracy(hip, oracle, dim, shape, method):
     assert drift < 1e-7
 
 
-@pytest.mark.parametrize("fusion,tile", [(2, 8), (4, 16), (1, 32)])
-def test_compensated_batch_equals_single_problem(hip, fusion, tile):
-    _announce("batch32c", fusion, tile)
+@pytest.mark.parametrize("kind,fusion,tile", [(0, 2, 8), (0, 4, 16), (0, 1, 32), (1, 2, 8), (1, 4, 12), (1, 1, 16)])
+def test_compensated_batch_equals_single_problem(hip, kind, fusion, tile):
+    _announce("batch32c", kind, fusion, tile)
     probs = _batch_inputs(3, 2)
     B = hip.Batch([p["shape"] for p in probs], [p["s"] for p in probs], [p["a"] for p in probs],
                   [p["h2"] for p in probs], [p["k"] for p in probs], method=2, precision=3, BC=1)
+    B.set_kernel(kind, tile)
     B.configure(fusion, tile, 512, 0)
     for i, p in enumerate(probs):
         B.set_state(i, p["U"], p["V"])
