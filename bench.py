@@ -43,7 +43,7 @@ CHUNK = 50
 K_RULE = 0.8 * 0.75 / (2 * np.sqrt(2))  # k / h^2, reference auto rule for 2SHOC 2D
 STAGE_BYTES = (56, 88, 88, 72)  # algorithmic HBM bytes per point, stages 1..4 (FP64)
 HBM_PEAK_GBS = 8000.0
-PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "fused2d_2shoc_f64_1024_traffic.json")
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "rows2d_2shoc_f64_1024_traffic.json")
 # HBM bytes per point and RK4 step that each fusion depth must move:
 #  1: four stage launches (56+88+88+72); 2: stages 1-2 read U,V write Y_2,ktot
 #  (56) then stages 3-4 read Y_2,U,V,ktot write U (72); 4: read U,V write U
