@@ -7,14 +7,15 @@
 //     a coalesced 1 KB (FP64) load from HBM and conflict-free 16 B per lane
 //     in LDS;
 //   - extended row e belongs to wave e mod 4 in every phase, which keeps the
-//     row's Y, D, F, U, V and ktot in that wave's registers; the rows are
-//     loaded all at once at the start (every load in flight together);
+//     row's Y, U, V and ktot in that wave's registers; the rows are loaded
+//     all at once at the start (every load in flight together);
 //   - j +- 1 neighbours of a row's own values come from adjacent lanes (DPP
 //     wave shifts), the rows i +- 1 (and the diagonal corners of the 2SHOC
 //     stencil) from the LDS copy of the stage input, and for 2SHOC the
 //     D = Lap_CD(Y)/h^2 rows i +- 1 from an LDS copy of D;
-//   - per stage: D phase, barrier, F phase, barrier, new Y rows written to
-//     LDS, barrier; the last stage writes its rows to HBM.
+//   - per stage: D phase, barrier, F phase (each row's F and RK4 update at
+//     once, the new Y in the row's registers), barrier, new Y rows written
+//     to LDS, barrier; the last stage writes its rows to HBM.
 // MSD boundary rows take their partner row's D/F from LDS after the barrier
 // that completes it (D: one extra barrier, only in tiles holding row 0 or
 // N-1); MSD boundary columns take the partner lane's values by DPP.  Every
@@ -31,6 +32,15 @@ namespace nlse {
 #ifndef ROWS_SCHED_FENCE
 #define ROWS_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
 #endif
+
+// Materialise a value here: an empty asm that takes and returns it in its
+// registers, so the optimiser cannot sink the arithmetic that produces it
+// past a later barrier (which would keep every row's LDS operands live
+// across the barrier).  No instruction is emitted.
+template <typename T> __device__ __forceinline__ void pin_here(c2<T>& v)
+{
+    asm volatile("" : "+v"(v.x), "+v"(v.y));
+}
 
 template <int METHOD, int FIRST, int LAST, int TI>
 struct RowsCfg {
@@ -105,7 +115,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
     };
 
     // rows of this wave: e = 4k + w; everything about row e stays in slot k
-    c2<T> Yc[RW], Uc[RW], Ac[RW], Dc[RW], Fc[RW], Lc[RW];
+    c2<T> Yc[RW], Uc[RW], Ac[RW], Lc[RW];
     T Vc[RW];
 
     // phase 0: every load of the tile in flight at once
@@ -168,7 +178,6 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 } else if (BC == BC_DIRICHLET) {
                     dd = dir_d<T>(yx, vx, c);
                 }
-                Dc[k] = dd;
                 sm.d[1 + (e - 1) * 64 + lane] = dd;
                 ROWS_SCHED_FENCE();
             }
@@ -182,15 +191,50 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                     if (g != 0 && g != N - 1) continue;
                     const int ep = (g == 0) ? e + 1 : e - 1;  // partner row
                     const T vp = V[(long)(g == 0 ? 1 : N - 2) * M + jc];
-                    const c2<T> dd = msd_d<T>(colq(sd(ep, 0)), colq(sy(ep, 0)), colq1(vp), Yc[k], Vc[k], c);
-                    Dc[k] = dd;
-                    sm.d[1 + (e - 1) * 64 + lane] = dd;
+                    sm.d[1 + (e - 1) * 64 + lane] = msd_d<T>(colq(sd(ep, 0)), colq(sy(ep, 0)), colq1(vp), Yc[k], Vc[k], c);
                 }
                 __syncthreads();
             }
         }
-        // F phase: rows into registers; the partner rows of MSD boundary rows
-        // also into LDS
+        // RK4 epilogue of row e (slot k): ktot and the next stage's input row
+        // go to registers (their LDS copy follows the barrier), or the last
+        // stage's result to HBM
+        const T cs = (QS == 3) ? c.k : c.k2;
+        auto epilogue = [&](int k, int e, c2<T> f) {
+            c2<T> a;
+            if (QS == 1) {
+                a = f;
+            } else {
+                const T wt = (QS == 4) ? (T)1.0 : (T)2.0;
+                a = mk<T>(Ac[k].x + wt * f.x, Ac[k].y + wt * f.y);
+            }
+            const c2<T> uo = Uc[k];
+            if (q < NF - 1) {
+                Ac[k] = a;
+                Yc[k] = mk<T>(uo.x + cs * f.x, uo.y + cs * f.y);
+                pin_here<T>(Ac[k]);
+                pin_here<T>(Yc[k]);
+            } else if (own) {  // the last stage's rows are the owned rows
+                const long p = (long)(g0 + e) * M + j;
+                if (QS == 4) {
+                    if constexpr (COMP) {
+                        c2<T> hi = uo, lo = Lc[k];
+                        comp_add<T>(hi.x, lo.x, c.k6 * a.x);
+                        comp_add<T>(hi.y, lo.y, c.k6 * a.y);
+                        ulo[p] = lo;
+                        out[p] = hi;
+                    } else {
+                        out[p] = mk<T>(uo.x + c.k6 * a.x, uo.y + c.k6 * a.y);
+                    }
+                } else {
+                    out[p] = mk<T>(uo.x + cs * f.x, uo.y + cs * f.y);
+                    acc[p] = a;
+                }
+            }
+        };
+        // F phase: each row's F and epilogue at once (a row's own Y is only
+        // read by its own wave); MSD boundary rows wait for the partner row's
+        // F, which goes to LDS
 #pragma unroll
         for (int k = 0; k < RW; ++k) {
             const int e = 4 * k + w;
@@ -202,7 +246,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
             if (interior(g)) {
                 const T n = nlin<T>(c.s, yx, vx);
                 if constexpr (METHOD == 2) {
-                    const c2<T> dc = Dc[k], da = sd(e + 1, 0), db = sd(e - 1, 0);
+                    const c2<T> dc = sd(e, 0), da = sd(e + 1, 0), db = sd(e - 1, 0);
                     const c2<T> dl = lane_shift2<DPP_NEXT, T>(dc), dr = lane_shift2<DPP_PREV, T>(dc);
                     const c2<T> aa = sy(e + 1, 1), ab = sy(e + 1, -1), ba = sy(e - 1, 1), bb = sy(e - 1, -1);
                     ff = mk<T>(-c.a * dc.y + c.a1_12 * (da.y + db.y + dl.y + dr.y)
@@ -225,10 +269,11 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 if (bot && g == N - 2) sm.y[1 + (TE - 1) * 64 + lane] = ff;
             } else if (BC == BC_LAP0) {
                 ff = lap0_f<T>(yx, vx, c.s);
-            } else if (msd) {
+            } else if (msd) {  // row 0 or N-1: F after the barrier
                 ypart[k] = sy(g == 0 ? e + 1 : e - 1, 0);
+                continue;
             }
-            Fc[k] = ff;
+            epilogue(k, e, ff);
             ROWS_SCHED_FENCE();
         }
         __syncthreads();  // every read of this stage's Y and D rows is done
@@ -238,46 +283,15 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 const int e = 4 * k + w;
                 if (e < flo || e >= fhi) continue;
                 const int g = g0 + e;
-                if (g == 0) Fc[k] = msd_f<T>(colq(sm.y[1 + lane]), colq(ypart[k]), Yc[k]);
-                else if (g == N - 1) Fc[k] = msd_f<T>(colq(sm.y[1 + (TE - 1) * 64 + lane]), colq(ypart[k]), Yc[k]);
+                if (g == 0) epilogue(k, e, msd_f<T>(colq(sm.y[1 + lane]), colq(ypart[k]), Yc[k]));
+                else if (g == N - 1) epilogue(k, e, msd_f<T>(colq(sm.y[1 + (TE - 1) * 64 + lane]), colq(ypart[k]), Yc[k]));
             }
         }
-        // RK4 epilogue of this stage's rows
-        const T cs = (QS == 3) ? c.k : c.k2;
+        if (q < NF - 1) {
 #pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const int e = 4 * k + w;
-            if (e < flo || e >= fhi) continue;
-            const c2<T> f = Fc[k];
-            c2<T> a;
-            if (QS == 1) {
-                a = f;
-            } else {
-                const T wt = (QS == 4) ? (T)1.0 : (T)2.0;
-                a = mk<T>(Ac[k].x + wt * f.x, Ac[k].y + wt * f.y);
-            }
-            const c2<T> uo = Uc[k];
-            if (q < NF - 1) {
-                Ac[k] = a;
-                const c2<T> yn = mk<T>(uo.x + cs * f.x, uo.y + cs * f.y);
-                Yc[k] = yn;
-                sm.y[1 + e * 64 + lane] = yn;
-            } else if (own) {  // the last stage's rows are the owned rows
-                const long p = (long)(g0 + e) * M + j;
-                if (QS == 4) {
-                    if constexpr (COMP) {
-                        c2<T> hi = uo, lo = Lc[k];
-                        comp_add<T>(hi.x, lo.x, c.k6 * a.x);
-                        comp_add<T>(hi.y, lo.y, c.k6 * a.y);
-                        ulo[p] = lo;
-                        out[p] = hi;
-                    } else {
-                        out[p] = mk<T>(uo.x + c.k6 * a.x, uo.y + c.k6 * a.y);
-                    }
-                } else {
-                    out[p] = mk<T>(uo.x + cs * f.x, uo.y + cs * f.y);
-                    acc[p] = a;
-                }
+            for (int k = 0; k < RW; ++k) {
+                const int e = 4 * k + w;
+                if (e >= flo && e < fhi) sm.y[1 + e * 64 + lane] = Yc[k];
             }
         }
         if (q < NF - 1) __syncthreads();  // the next stage's input rows are in LDS
