@@ -42,12 +42,24 @@ H = 0.1
 CHUNK = 50
 K_RULE = 0.8 * 0.75 / (2 * np.sqrt(2))  # k / h^2, reference auto rule for 2SHOC 2D
 STAGE_BYTES = (56, 88, 88, 72)  # algorithmic HBM bytes per point, stages 1..4 (FP64)
+
+
+def stage_bytes(precision):
+    """Algorithmic HBM bytes per point of stages 1..4 for complex elements of
+    e bytes and a real V of v bytes: 3e+v, 5e+v, 5e+v, 4e+v, plus the low
+    part of a compensated state read and written by stage 4."""
+    e, v = (16, 8) if precision == 2 else (8, 4)
+    lo = 2 * e if precision == 3 else 0
+    return (3 * e + v, 5 * e + v, 5 * e + v, 4 * e + v + lo)
+
+
+def design_bytes(precision, fusion):
+    """HBM bytes per point and RK4 step of 1, 2 or 4 stages per launch."""
+    e, v = (16, 8) if precision == 2 else (8, 4)
+    lo = 2 * e if precision == 3 else 0
+    return {1: sum(stage_bytes(precision)), 2: 7 * e + 2 * v + lo, 4: 2 * e + v + lo}[fusion]
 HBM_PEAK_GBS = 8000.0
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "rows2d_2shoc_f64_1024_traffic.json")
-# HBM bytes per point and RK4 step that each fusion depth must move:
-#  1: four stage launches (56+88+88+72); 2: stages 1-2 read U,V write Y_2,ktot
-#  (56) then stages 3-4 read Y_2,U,V,ktot write U (72); 4: read U,V write U
-DESIGN_BYTES = {1: 304, 2: 128, 4: 40}
 KERNEL_FAMILY = {0: "fused2d_kernel", 1: "rows2d_kernel", 2: "strip2d_kernel"}
 
 
@@ -137,8 +149,12 @@ def run_batched(args):
         states = [batched_state(p) for p in probs]
         for i, (U, V) in enumerate(states):
             B.set_state(i, U, V)
-        B.autotune(3)
-        f, kind, t, g = B.config()
+        if args.batch_config:
+            f, kind, t, g = (int(x) for x in args.batch_config.split(","))
+            B.set_kernel(kind, t)
+        else:
+            B.autotune(3)
+            f, kind, t, g = B.config()
         B.configure(f, t, g, chunk)
         B.step(chunk)  # warm-up: captures the chunk graph
         B.sync()
@@ -326,10 +342,14 @@ def main():
     ap.add_argument("--graph", type=int, default=CHUNK, help="RK4 steps per hipGraph replay (0 = off)")
     ap.add_argument("--fusion", type=int, default=2, choices=(1, 2, 4),
                     help="RK4 stages per kernel launch (results are identical)")
+    ap.add_argument("--precision", type=int, default=2, choices=(1, 2, 3),
+                    help="2d workload: 2 FP64 (the headline), 1 FP32, 3 compensated FP32")
     ap.add_argument("--kernel", type=int, default=1, choices=(0, 1, 2),
                     help="2D stage kernel: 0 LDS tiles, 1 row-sweep tiles, 2 register strips")
     ap.add_argument("--tile", type=int, default=8, help="rows per tile (strip segment) of that kernel")
     ap.add_argument("--autotune", action="store_true", help="pick fusion and tile with the built-in tuner")
+    ap.add_argument("--batch-config", default="", metavar="F,KIND,ROWS,GRID",
+                    help="batched: stages per launch, kernel kind, rows, grid (skips the tuner)")
     ap.add_argument("--qps", type=float, default=4.0, help="serving: Poisson arrival rate")
     ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
     ap.add_argument("--workload", default="2d", choices=("2d", "batched", "1d", "3d", "serving"),
@@ -363,7 +383,7 @@ def main():
 
     U, V = initial_state()
     k = K_RULE * H * H
-    sol = nm.Solver((N, M), 2, 2, 2, -1.0, 1.0, H * H, k)
+    sol = nm.Solver((N, M), 2, args.precision, 2, -1.0, 1.0, H * H, k)
     sol.set_graph(args.graph)
     sol.set_fusion(args.fusion)
     sol.set_kernel(args.kernel, args.tile)
@@ -390,11 +410,12 @@ def main():
     # time over the launch count is the mean launch duration
     nlaunch = total_steps * (4 // args.fusion)
     t_launch = dev_ms * 1e-3 / nlaunch
-    bytes_launch = sum(STAGE_BYTES) * N * M / (4 // args.fusion)
+    alg_bytes = sum(stage_bytes(args.precision))
+    bytes_launch = alg_bytes * N * M / (4 // args.fusion)
     achieved = bytes_launch / t_launch / 1e9
-    design_bytes = DESIGN_BYTES[args.fusion]
+    dbytes = design_bytes(args.precision, args.fusion)
     traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
+    if os.path.exists(PROFILE_TRAFFIC) and args.precision == 2:
         with open(PROFILE_TRAFFIC) as f:
             prof = json.load(f).get(f"fusion{args.fusion}", {})
         if prof.get("tile") == args.tile and prof.get("kernel", 0) == args.kernel:
@@ -403,15 +424,15 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kernel_name(args.fusion, args.kernel, args.tile),
                 "launch_us": round(t_launch * 1e6, 2),
-                "algorithmic_bytes_per_point_step": sum(STAGE_BYTES),
-                "design_bytes_per_point_step": design_bytes,
-                "design_GBps": round(design_bytes * N * M / (t_launch * (4 // args.fusion)) / 1e9, 1),
-                "design_frac": round(design_bytes * N * M / (t_launch * (4 // args.fusion)) / 1e9 / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_point_step": alg_bytes,
+                "design_bytes_per_point_step": dbytes,
+                "design_GBps": round(dbytes * N * M / (t_launch * (4 // args.fusion)) / 1e9, 1),
+                "design_frac": round(dbytes * N * M / (t_launch * (4 // args.fusion)) / 1e9 / HBM_PEAK_GBS, 4),
                 "launch_us_event_per_kernel": [round(x * 1e3, 2) for x in sol.profile_launches(20)]}
     sol.close()
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.precision == 2:
         cpu = cpu_baseline(U, V, k)
 
     line = {
@@ -425,7 +446,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {2: "f64", 1: "f32", 3: "f32 (compensated hi+lo state)"}[args.precision],
         "data": "synthetic",
         "config": {"workload": "2D 2SHOC RK4 FP64, 1024x1024, MSD, dark vortex + 1e-3 noise (seed 0)",
                    "grid": [N, M], "h": H, "k_over_h2": round(K_RULE, 6), "chunk_size": CHUNK,
