@@ -126,7 +126,10 @@ int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
  *           8 (default), 12 or 16 rows
  *   kind 2  register-marching strips (one wave per strip, no LDS), segments
  *           of 3..256 rows, for 1 or 2 stages per launch (4 stages per launch
- *           run on kind 1 with 8 rows) */
+ *           run on kind 1 with 8 rows)
+ * 3D 2SHOC: kind 1 (default) marches planes with D kept on chip, tiles of 4
+ * (default) or 8 rows x 60 columns; kind 0 runs a D pass and an F pass per
+ * stage.  MSD boundaries always use the two passes.  1D: ignored. */
 int nlse_set_kernel(nlse_solver *h, int kind, int rows);
 /* nlse_set_kernel(h, 0, tile_rows) */
 int nlse_set_tile(nlse_solver *h, int tile_rows);

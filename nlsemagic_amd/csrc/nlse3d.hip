@@ -5,11 +5,16 @@
 // :308-322 / :507-521 on the host.  Element (i, j, k) is at (i*N + j)*M + k:
 // i < L slowest, k < M contiguous.
 //   CD:    one fused kernel per RK4 stage (7-point stencil + BC + update).
-//   2SHOC: per stage, a D pass (D = Lap_CD(Y)/h^2 with its boundary values,
-//          written to a scratch field) and a fused F + update pass that reads
-//          D's six neighbours and Y's twelve edge neighbours.
-// Neighbour reads go through the L1/L2 caches: a wavefront covers 64
-// consecutive k, so every plane and row neighbour is a coalesced line read.
+//   2SHOC: one plane-marching kernel per RK4 stage that keeps D on chip
+//          (nlse3d_march.h); for MSD boundaries, whose partner can lie in
+//          another plane and row, a D pass (D = Lap_CD(Y)/h^2 with its
+//          boundary values, written to a scratch field) and a fused F +
+//          update pass that reads D's six neighbours and Y's twelve edge
+//          neighbours.
+// Neighbour reads of the per-point kernels go through the L1/L2 caches: a
+// wavefront covers 64 consecutive k, so every plane and row neighbour is a
+// coalesced line read.
+#include "nlse3d_march.h"
 #include "nlse_common.h"
 #include "nlse_launch.h"
 
@@ -123,9 +128,46 @@ __global__ __launch_bounds__(256) void f3_kernel(const c2<T>* __restrict__ y, co
     rk4_epilogue<T>(stage, c, f, uu, acc, yout, yout, p, ulo);
 }
 
+// 2SHOC stage with D on chip: one workgroup per (j tile, k tile, i chunk);
+// consecutive tiles of a chunk go to the same XCD
+template <typename T, int TJ>
+__global__ __launch_bounds__(256) void march3d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                      c2<T>* __restrict__ acc, c2<T>* yout, const T* __restrict__ V,
+                                                      Coefs<T> c, March3 g, int BC, int stage, c2<T>* ulo)
+{
+    __shared__ March3Lds<T, TJ> sm;
+    const int t = (blockIdx.x & 7) * g.per_xcd + (blockIdx.x >> 3);
+    if (t >= g.tj * g.tk * g.ti) return;
+    const int kt = t % g.tk, rest = t / g.tk;
+    const int jt = rest % g.tj, ci = rest / g.tj;
+    march3d_tile<T, TJ>(sm, ci * g.IC, jt * TJ, kt * March3Cfg<TJ>::W, g, yin, u, acc, yout, V, c, BC, stage, ulo);
+}
+
+template <typename T, int TJ>
+static hipError_t launch_march3d(const StageArgs<T>& a, int IC, hipStream_t st)
+{
+    March3 g;
+    g.N = a.N;
+    g.M = a.M;
+    g.L = a.L;
+    g.IC = IC;
+    g.tj = (a.N + TJ - 1) / TJ;
+    g.tk = (a.M + March3Cfg<TJ>::W - 1) / March3Cfg<TJ>::W;
+    g.ti = (a.L + IC - 1) / IC;
+    g.per_xcd = (g.tj * g.tk * g.ti + 7) / 8;
+    hipLaunchKernelGGL((march3d_kernel<T, TJ>), dim3(8 * g.per_xcd), dim3(256), 0, st, a.yin, a.u, a.acc, a.yout,
+                       a.V, a.c, g, a.BC, a.stage, a.ulo);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_stage3d(const StageArgs<T>& a, hipStream_t st)
 {
+    if (a.method == 2 && a.BC != BC_MSD && a.kernel != KERNEL_TILES) {
+        // rows per tile: 4 or 8 (a.tile below 8 selects 4); planes per chunk 32
+        if (a.tile < 8) return launch_march3d<T, 4>(a, 32, st);
+        return launch_march3d<T, 8>(a, 32, st);
+    }
     Grid3 g;
     g.N = a.N;
     g.M = a.M;

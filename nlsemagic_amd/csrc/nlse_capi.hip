@@ -410,8 +410,8 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
     }
     h->ubase = h->U;
     h->fuse = (dim == 2) ? 2 : 1;
-    h->kern = KERNEL_ROWS;  // the fastest family on a 1024^2 grid
-    h->tile = 8;
+    h->kern = KERNEL_ROWS;         // the fastest family on a 1024^2 grid and for 3D 2SHOC
+    h->tile = (dim == 3) ? 4 : 8;  // 3D: plane-marching tiles of 4 rows
     set_params(h, BC, s, a, h2, k);
     // a defined initial state: zero field and potential
     (void)hipMemsetAsync(h->U, 0, eb, h->stream);
@@ -567,7 +567,12 @@ int nlse_set_fusion(nlse_solver* h, int stages_per_launch)
 int nlse_set_kernel(nlse_solver* h, int kind, int rows)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");
-    if (kind == KERNEL_TILES) {
+    if (h->dim == 3) {
+        if (kind == KERNEL_ROWS && rows != 4 && rows != 8)
+            return fail(NLSE_EINVAL, "3D plane-marching tile rows must be 4 or 8");
+        if (kind != KERNEL_ROWS && kind != KERNEL_TILES)
+            return fail(NLSE_EINVAL, "3D kernel kind must be 0 (per-point passes) or 1 (plane-marching 2SHOC)");
+    } else if (kind == KERNEL_TILES) {
         if (rows != 8 && rows != 16 && rows != 32) return fail(NLSE_EINVAL, "LDS tile rows must be 8, 16 or 32");
     } else if (kind == KERNEL_ROWS) {
         if (rows != 8 && rows != 12 && rows != 16) return fail(NLSE_EINVAL, "row-sweep tile rows must be 8, 12 or 16");

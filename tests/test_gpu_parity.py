@@ -279,3 +279,46 @@ def test_kernel_families_compensated_agree(hip, method, fusion):
             res.append(sol.state())
             sol.close()
         assert bits_equal(res[0], res[1]) and bits_equal(res[0], res[2])
+
+
+def _march3d_cases():
+    """3D 2SHOC plane-marching stage kernel: several i chunks (32 planes),
+    j tiles (4 or 8 rows) and k tiles (60 columns), a last k tile of one or
+    two columns, one plane/row/column of interior only; Dirichlet, Lap0 and
+    the zero boundary (MSD runs on the two-pass kernels)."""
+    out = []
+    seed = 20000
+    for shape in [(70, 20, 130), (33, 9, 61), (40, 17, 121), (3, 30, 64), (65, 3, 3), (34, 11, 62)]:
+        for prec in (2, 1):
+            for BC in (1, 3, 0):
+                for rows in (4, 8):
+                    seed += 1
+                    out.append((shape, prec, BC, rows, seed))
+    return out
+
+
+@pytest.mark.parametrize("shape,prec,BC,rows,seed", _march3d_cases())
+def test_march3d_matches_oracle_and_two_pass(hip, oracle, shape, prec, BC, rows, seed):
+    """The plane-marching 3D 2SHOC kernel (D on chip) gives the reference's
+    bits, and the same bits as the two-pass kernels."""
+    _announce("march3d", shape, prec, BC, rows, seed)
+    rng = np.random.default_rng(seed)
+    U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
+    V = 0.5 * rng.uniform(-1, 1, shape)
+    h2 = 0.04
+    k = 0.1 * h2
+    s = -1.0 if BC != 3 else 0.8
+    exp = oracle.take_steps(U, V, s, 0.9, h2, BC, 3, k, 2, precision=prec)
+    res = []
+    for kind in (1, 0):
+        sol = hip.Solver(shape, 2, prec, BC, s, 0.9, h2, k)
+        sol.set_kernel(kind, rows if kind == 1 else 8)
+        sol.set_state(U, V)
+        sol.step(3)
+        got = sol.state()
+        sol.close()
+        if prec == 1:
+            got = got.astype(np.complex64).astype(np.complex128)
+        res.append(got)
+    assert bits_equal(res[0], exp), f"max |diff| {np.max(np.abs(res[0] - exp))}"
+    assert bits_equal(res[1], exp)
