@@ -254,8 +254,8 @@ def run_grid(args):
     for m in methods:
         for prec in (2, 1, 3):
             sol = nm.Solver(shape, m, prec, BC, -1.0, 1.0, h * h, ks[m])
-            if args.workload == "3d" and (args.kernel, args.tile) != (1, 8):
-                sol.set_kernel(args.kernel, args.tile)  # default: plane-marching tiles of 4 rows
+            if args.workload == "3d" and (args.kernel != 1 or args.tile is not None):
+                sol.set_kernel(args.kernel, args.tile or 8)
             sol.set_state(U, V)
             sol.step(chunk)
             sol.sync()
@@ -348,8 +348,9 @@ def main():
                     help="2d workload: 2 FP64 (the headline), 1 FP32, 3 compensated FP32")
     ap.add_argument("--kernel", type=int, default=1, choices=(0, 1, 2),
                     help="2D stage kernel: 0 LDS tiles, 1 row-sweep tiles, 2 register strips")
-    ap.add_argument("--tile", type=int, default=8,
-                    help="rows per tile (strip segment) of that kernel; 3d: 4 or 8 rows of the plane-marching tiles")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="rows per tile (strip segment) of that kernel (2d default 8); "
+                         "3d: 4 (default) or 8 rows of the plane-marching tiles")
     ap.add_argument("--autotune", action="store_true", help="pick fusion and tile with the built-in tuner")
     ap.add_argument("--batch-config", default="", metavar="F,KIND,ROWS,GRID",
                     help="batched: stages per launch, kernel kind, rows, grid (skips the tuner)")
@@ -389,6 +390,8 @@ def main():
     sol = nm.Solver((N, M), 2, args.precision, 2, -1.0, 1.0, H * H, k)
     sol.set_graph(args.graph)
     sol.set_fusion(args.fusion)
+    if args.tile is None:
+        args.tile = 8
     sol.set_kernel(args.kernel, args.tile)
     sol.set_state(U, V)
     if args.autotune:
