@@ -190,6 +190,12 @@ def run_batched(args):
         "config": {"workload": "batched: 512 uneven 2D 2SHOC problems (sides 64-512, seed 1), chunks of 100 steps",
                    "points": npts, "fp64_rate": res[2]["rate"], "fp32c_over_fp64": res[3]["rate"] / res[2]["rate"],
                    "fp32c_rel_l2_max": float(rel.max()), "fp32c_rel_l2_p50": pct(rel, 50),
+                   "fp32c_rel_l2_p99": pct(rel, 99),
+                   "fp32c_problems_rel_l2_le_1e-6": f"{int(np.sum(rel <= 1e-6))}/{len(rel)}",
+                   "fp32c_worst_problem": {"index": int(np.argmax(rel)), **{k: probs[int(np.argmax(rel))][k]
+                                                                           for k in ("n", "s", "a", "w")},
+                                           "max_abs_V": float(probs[int(np.argmax(rel))]["w"] ** 2 * 2 *
+                                                              (H * probs[int(np.argmax(rel))]["n"] / 2) ** 2)},
                    "fp32c_mass_vs_fp64_max": float(np.abs(mass_rel).max()),
                    "fp64_mass_drift_max": float(np.abs(res[2]["mass_drift"]).max()),
                    "fp32c_mass_drift_max": float(np.abs(res[3]["mass_drift"]).max()),
