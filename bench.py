@@ -260,7 +260,7 @@ def run_grid(args):
     for m in methods:
         for prec in (2, 1, 3):
             sol = nm.Solver(shape, m, prec, BC, -1.0, 1.0, h * h, ks[m])
-            if args.workload == "3d" and (args.kernel != 1 or args.tile is not None):
+            if args.kernel != 1 or args.tile is not None:  # 3d: plane-march rows; 1d: steps per launch
                 sol.set_kernel(args.kernel, args.tile or 8)
             sol.set_state(U, V)
             sol.step(chunk)

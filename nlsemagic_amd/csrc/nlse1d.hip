@@ -10,8 +10,17 @@
 //     N <= 4096; U and ktot stay in registers, the stage input and D live in
 //     LDS, and the four stages of every step are separated by barriers only.
 //     This is the launch-free form for the small 1D problems the reference
-//     targets (N = 2048): a grid-wide launch per stage would be latency bound.
+//     targets (N = 2048): a grid-wide launch per stage would be latency bound;
+//   - block1d_kernel (the default): S whole RK4 steps per launch, one
+//     workgroup per segment of 256 points, each on its own CU.  A workgroup
+//     loads its segment plus a halo of 4*R*S points (R = 1 CD, 2 2SHOC: the
+//     reach of one stage), runs the S steps on chip like the resident kernel,
+//     and writes back its own points; values near the window edge go stale
+//     by R points per stage and never reach the owned points.  Launches
+//     alternate between two state buffers.
 // Arithmetic follows the reference operand order with FP contraction off.
+#include <algorithm>
+
 #include "nlse_common.h"
 #include "nlse_launch.h"
 
@@ -247,6 +256,110 @@ static hipError_t launch_res(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c, int N
     return hipGetLastError();
 }
 
+// ------------------------------------------ time-blocked windows, many CUs
+
+constexpr int BLK_SEG = 256;  // owned points per workgroup
+
+template <typename T, int METHOD, bool COMP>
+__global__ __launch_bounds__(1024) void block1d_kernel(const c2<T>* __restrict__ uin, c2<T>* __restrict__ uout,
+                                                       const c2<T>* __restrict__ loin, c2<T>* __restrict__ loout,
+                                                       const T* __restrict__ Vg, Coefs<T> c, int N, int BC,
+                                                       int steps, int halo)
+{
+    __shared__ c2<T> sY[1024 + 2];
+    __shared__ c2<T> sD[(METHOD == 2) ? 1024 + 2 : 1];
+    const int o0 = blockIdx.x * BLK_SEG, o1 = min(o0 + BLK_SEG, N);
+    const int w0 = max(o0 - halo, 0), w1 = min(o1 + halo, N);
+    const int t = threadIdx.x;
+    const int i = w0 + t;
+    const bool act = i < w1;
+    const int ic = act ? i : w1 - 1;
+    c2<T> U = uin[ic], Lo = COMP ? loin[ic] : mk<T>(0, 0), A = mk<T>(0, 0), Y = U;
+    const T Vr = Vg[ic];
+    // window-local accessors: one padding element each side (stale values
+    // beyond the window only reach points within the halo)
+    auto Yl = [&](int gi) { return sY[gi - w0 + 1]; };
+    auto Vl = [&](int gi) { return gi == i ? Vr : Vg[gi]; };
+    auto Dl = [&](int gi) { return sD[gi - w0 + 1]; };
+    auto rhs = make_rhs1d<T, METHOD>(Yl, Vl, Dl, c, N, BC);
+    if (t == 0) {
+        sY[0] = mk<T>(0, 0);
+        if (METHOD == 2) sD[0] = mk<T>(0, 0);
+    }
+    if (t == w1 - w0) {
+        sY[t + 1] = mk<T>(0, 0);
+        if (METHOD == 2) sD[t + 1] = mk<T>(0, 0);
+    }
+    for (int step = 0; step < steps; ++step) {
+#pragma unroll 1
+        for (int stage = 1; stage <= 4; ++stage) {
+            if (act) sY[t + 1] = Y;
+            __syncthreads();
+            if (METHOD == 2) {
+                if (act) sD[t + 1] = rhs.d_at(i);
+                __syncthreads();
+            }
+            if (act) {
+                const c2<T> f = rhs.f_at(i);
+                if (stage == 1) {
+                    A = f;
+                    Y = mk<T>(U.x + c.k2 * f.x, U.y + c.k2 * f.y);
+                } else if (stage == 4) {
+                    A.x = A.x + (T)1.0 * f.x;
+                    A.y = A.y + (T)1.0 * f.y;
+                    if constexpr (COMP) {
+                        comp_add<T>(U.x, Lo.x, c.k6 * A.x);
+                        comp_add<T>(U.y, Lo.y, c.k6 * A.y);
+                    } else {
+                        U = mk<T>(U.x + c.k6 * A.x, U.y + c.k6 * A.y);
+                    }
+                    Y = U;
+                } else {
+                    A.x = A.x + (T)2.0 * f.x;
+                    A.y = A.y + (T)2.0 * f.y;
+                    const T kk = (stage == 2) ? c.k2 : c.k;
+                    Y = mk<T>(U.x + kk * f.x, U.y + kk * f.y);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (act && i >= o0 && i < o1) {
+        uout[i] = U;
+        if (COMP) loout[i] = Lo;
+    }
+}
+
+int block1d_max_steps(int method) { return (1024 - BLK_SEG) / (2 * 4 * (method == 2 ? 2 : 1)); }
+
+template <typename T>
+hipError_t launch_block1d(const c2<T>* uin, c2<T>* uout, const c2<T>* loin, c2<T>* loout, const T* V, Coefs<T> c,
+                          int N, int BC, int method, int steps, int halo_steps, hipStream_t st)
+{
+    const int R = (method == 2) ? 2 : 1;
+    const int halo = 4 * R * halo_steps;
+    const int nb = (N + BLK_SEG - 1) / BLK_SEG;
+    const int win = std::min(N, BLK_SEG + 2 * halo);
+    const dim3 blk(((win + 63) / 64) * 64);
+    const bool comp = loin != nullptr;
+    if (method == 2) {
+        if (comp)
+            hipLaunchKernelGGL((block1d_kernel<T, 2, true>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
+                               BC, steps, halo);
+        else
+            hipLaunchKernelGGL((block1d_kernel<T, 2, false>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
+                               BC, steps, halo);
+    } else {
+        if (comp)
+            hipLaunchKernelGGL((block1d_kernel<T, 1, true>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
+                               BC, steps, halo);
+        else
+            hipLaunchKernelGGL((block1d_kernel<T, 1, false>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
+                               BC, steps, halo);
+    }
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_chunk1d_resident(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c, int N, int BC, int method,
                                    int steps, hipStream_t st)
@@ -258,6 +371,10 @@ hipError_t launch_chunk1d_resident(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c,
 
 template hipError_t launch_stage1d<double>(const StageArgs<double>&, hipStream_t);
 template hipError_t launch_stage1d<float>(const StageArgs<float>&, hipStream_t);
+template hipError_t launch_block1d<double>(const double2*, double2*, const double2*, double2*, const double*,
+                                           Coefs<double>, int, int, int, int, int, hipStream_t);
+template hipError_t launch_block1d<float>(const float2*, float2*, const float2*, float2*, const float*, Coefs<float>,
+                                          int, int, int, int, int, hipStream_t);
 template hipError_t launch_chunk1d_resident<double>(double2*, double2*, const double*, Coefs<double>, int, int,
                                                     int, int, hipStream_t);
 template hipError_t launch_chunk1d_resident<float>(float2*, float2*, const float*, Coefs<float>, int, int, int,

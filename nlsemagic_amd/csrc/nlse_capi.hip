@@ -109,9 +109,14 @@ void set_params(nlse_solver* h, int BC, double s, double a, double h2, double k)
     drop_graphs(h);
 }
 
+// 1D: time-blocked windows of h->tile steps per launch (kind 1, default), or
+// (kind 0) the whole chunk in one workgroup for N <= 4096, per-stage
+// launches above
+bool blocked1d(const nlse_solver* h) { return h->dim == 1 && h->kern == KERNEL_ROWS; }
+
 bool resident1d(const nlse_solver* h)
 {
-    return h->dim == 1 && h->N <= chunk1d_resident_max_points((int)elem_bytes(h->prec));
+    return h->dim == 1 && !blocked1d(h) && h->N <= chunk1d_resident_max_points((int)elem_bytes(h->prec));
 }
 
 template <typename T>
@@ -204,6 +209,22 @@ int enqueue_steps_t(nlse_solver* h, const Coefs<T>& c, int steps)
 {
     using C = c2<T>;
     if (steps <= 0) return NLSE_OK;
+    if (blocked1d(h)) {
+        // an even number of launches (for two or more steps), so U ends in
+        // the buffer it started in and graph replays need no pointer fix-up
+        const int S = h->tile;
+        int nl = (steps + S - 1) / S;
+        if (nl > 1 && (nl & 1)) ++nl;
+        if (nl == 1 && steps > 1) nl = 2;
+        for (int l = 0; l < nl; ++l) {
+            const int n = steps / nl + (l < steps % nl ? 1 : 0);
+            HIPCHK(launch_block1d<T>((const C*)h->U, (C*)h->Y1, (const C*)h->Ulo, (C*)h->Y0, (const T*)h->V, c, h->N,
+                                     h->BC, h->method, n, S, h->stream));
+            std::swap(h->U, h->Y1);
+            if (h->Ulo) std::swap(h->Ulo, h->Y0);
+        }
+        return NLSE_OK;
+    }
     if (resident1d(h)) {
         HIPCHK(launch_chunk1d_resident<T>((C*)h->U, (C*)h->Ulo, (const T*)h->V, c, h->N, h->BC, h->method, steps,
                                           h->stream));
@@ -230,6 +251,7 @@ int step_impl(nlse_solver* h, int steps)
     // one per buffer holding U when the replay starts
     int gs = h->gsteps;
     if (fusion(h) > 1 && (gs & 1)) gs += 1;
+    if (blocked1d(h) && gs == 1) gs = 2;  // one launch would leave U in the other buffer
     if (gs <= 0 || resident1d(h) || steps < gs) return enqueue_steps(h, steps);
     const int slot = (h->U == h->ubase) ? 0 : 1;
     if (!h->gexec[slot] || h->gcaptured[slot] != gs) {
@@ -238,11 +260,15 @@ int step_impl(nlse_solver* h, int steps)
         hipGraph_t g;
         void* u_before = h->U;
         void* y1_before = h->Y1;
+        void* lo_before = h->Ulo;
+        void* y0_before = h->Y0;
         HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
         int rc = enqueue_steps(h, gs);
         hipError_t e = hipStreamEndCapture(h->stream, &g);
         h->U = u_before;  // capture records launches without running them
         h->Y1 = y1_before;
+        h->Ulo = lo_before;
+        h->Y0 = y0_before;
         if (rc != NLSE_OK) return rc;
         if (e != hipSuccess) return fail(NLSE_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
         e = hipGraphInstantiate(&h->gexec[slot], g, nullptr, nullptr, 0);
@@ -411,7 +437,7 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
     h->ubase = h->U;
     h->fuse = (dim == 2) ? 2 : 1;
     h->kern = KERNEL_ROWS;         // the fastest family on a 1024^2 grid and for 3D 2SHOC
-    h->tile = (dim == 3) ? 4 : 8;  // 3D: plane-marching tiles of 4 rows
+    h->tile = (dim == 3) ? 4 : (dim == 1 ? 16 : 8);  // 3D: plane-march rows; 1D: steps per launch
     set_params(h, BC, s, a, h2, k);
     // a defined initial state: zero field and potential
     (void)hipMemsetAsync(h->U, 0, eb, h->stream);
@@ -497,6 +523,7 @@ int nlse_profile_launches(nlse_solver* h, int steps, double* ms_launch, int* nla
 {
     if (!h || !ms_launch || !nlaunch || steps <= 0) return fail(NLSE_EINVAL, "bad argument");
     if (resident1d(h)) return fail(NLSE_EINVAL, "small 1D problems run whole chunks in one kernel");
+    if (blocked1d(h)) return fail(NLSE_EINVAL, "1D time-blocked windows run several steps per launch");
     const int nl = 4 / fusion(h);
     hipEvent_t ev[5];
     for (int i = 0; i <= nl; ++i) HIPCHK(hipEventCreate(&ev[i]));
@@ -567,7 +594,13 @@ int nlse_set_fusion(nlse_solver* h, int stages_per_launch)
 int nlse_set_kernel(nlse_solver* h, int kind, int rows)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");
-    if (h->dim == 3) {
+    if (h->dim == 1) {
+        if (kind == KERNEL_ROWS && (rows < 1 || rows > block1d_max_steps(h->method)))
+            return fail(NLSE_EINVAL, "1D time-blocked windows take 1.." + std::to_string(block1d_max_steps(h->method)) +
+                                         " steps per launch");
+        if (kind != KERNEL_ROWS && kind != KERNEL_TILES)
+            return fail(NLSE_EINVAL, "1D kernel kind must be 0 (one workgroup / per-stage) or 1 (time-blocked)");
+    } else if (h->dim == 3) {
         if (kind == KERNEL_ROWS && rows != 4 && rows != 8)
             return fail(NLSE_EINVAL, "3D plane-marching tile rows must be 4 or 8");
         if (kind != KERNEL_ROWS && kind != KERNEL_TILES)

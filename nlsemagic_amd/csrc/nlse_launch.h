@@ -75,4 +75,12 @@ hipError_t launch_chunk1d_resident(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c,
                                    int steps, hipStream_t st);
 int chunk1d_resident_max_points(int elem_bytes);
 
+// `steps` RK4 steps of a 1D problem per launch on time-blocked windows (one
+// workgroup per 256 points, halo for halo_steps >= steps); reads uin/loin and
+// writes uout/loout (loin == nullptr: no compensated low part).
+template <typename T>
+hipError_t launch_block1d(const c2<T>* uin, c2<T>* uout, const c2<T>* loin, c2<T>* loout, const T* V, Coefs<T> c,
+                          int N, int BC, int method, int steps, int halo_steps, hipStream_t st);
+int block1d_max_steps(int method);
+
 }  // namespace nlse

@@ -322,3 +322,48 @@ def test_march3d_matches_oracle_and_two_pass(hip, oracle, shape, prec, BC, rows,
         res.append(got)
     assert bits_equal(res[0], exp), f"max |diff| {np.max(np.abs(res[0] - exp))}"
     assert bits_equal(res[1], exp)
+
+
+def _block1d_cases():
+    """1D time-blocked windows: grids smaller than, equal to and larger than
+    one 256-point segment, steps per launch 1..max (CD 96, 2SHOC 48), chunk
+    lengths that split into odd and even launch counts, every BC."""
+    out = []
+    seed = 30000
+    for n in (5, 256, 257, 700, 2048, 5000):
+        for method in (1, 2):
+            for BC in (1, 2, 3, 4, 0):
+                seed += 1
+                S = (1, 3, 8, 17, 48 if method == 2 else 96)[seed % 5]
+                steps = (1, 5, 37, 100)[seed % 4]
+                out.append((n, method, BC, S, steps, (2, 1, 3)[seed % 3], seed))
+    return out
+
+
+@pytest.mark.parametrize("n,method,BC,S,steps,prec,seed", _block1d_cases())
+def test_block1d_matches_oracle(hip, oracle, n, method, BC, S, steps, prec, seed):
+    """Time-blocked 1D windows give the reference's bits (compensated: the
+    same bits as the one-workgroup kernel)."""
+    _announce("block1d", n, method, BC, S, steps, prec, seed)
+    rng = np.random.default_rng(seed)
+    U = (0.9 + 0.3 * rng.uniform(-1, 1, n)) + 1j * (0.3 * rng.uniform(-1, 1, n))
+    V = 0.5 * rng.uniform(-1, 1, n)
+    h2 = 0.04
+    k = 0.1 * h2
+    s = -1.0 if BC != 3 else 0.8
+    res = []
+    for kind, rows, graph in ((1, S, 0), (1, S, 2 * S + 1), (0, 8, 0)):
+        sol = hip.Solver((n,), method, prec, BC, s, 0.9, h2, k)
+        sol.set_kernel(kind, rows)
+        sol.set_graph(graph)
+        sol.set_state(U, V)
+        sol.step(steps)
+        sol.step(steps)
+        res.append(sol.state())
+        sol.close()
+    if prec != 3:
+        exp = oracle.take_steps(U, V, s, 0.9, h2, BC, 2 * steps, k, method, precision=prec)
+        if prec == 1:
+            res = [r.astype(np.complex64).astype(np.complex128) for r in res]
+        assert bits_equal(res[0], exp), f"max |diff| {np.max(np.abs(res[0] - exp))}"
+    assert bits_equal(res[0], res[1]) and bits_equal(res[0], res[2])
