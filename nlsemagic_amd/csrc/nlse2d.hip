@@ -94,8 +94,8 @@ static hipError_t launch_strip(const StageArgs<T>& a, hipStream_t st)
 }
 
 // Row-sweep LDS tiles (nlse2d_rows.h): one workgroup per tile of TI rows.
-template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP>
-__global__ __launch_bounds__(256) void rows2d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP, int NW>
+__global__ __launch_bounds__(NW * 64) void rows2d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
                                                      c2<T>* __restrict__ acc, c2<T>* out, const T* __restrict__ V,
                                                      Coefs<T> c, Grid2 g, int BC, c2<T>* __restrict__ ulo)
 {
@@ -106,9 +106,10 @@ __global__ __launch_bounds__(256) void rows2d_kernel(const c2<T>* __restrict__ y
     const int ti = t / g.tiles_j, tj = t - ti * g.tiles_j;
     const int i0 = ti * TI - g.ioff, j0 = tj * W - g.joff;
     if (rows_inner<METHOD, FIRST, LAST, TI>(i0, j0, g.N, g.M))
-        rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true>(sm, i0, j0, g.N, g.M, yin, u, acc, out, V, c, BC, ulo);
+        rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true, NW>(sm, i0, j0, g.N, g.M, yin, u, acc, out, V, c, BC, ulo);
     else
-        rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, false>(sm, i0, j0, g.N, g.M, yin, u, acc, out, V, c, BC, ulo);
+        rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, false, NW>(sm, i0, j0, g.N, g.M, yin, u, acc, out, V, c, BC,
+                                                                 ulo);
 }
 
 template <typename T, int METHOD, int FIRST, int LAST, int TI>
@@ -124,15 +125,18 @@ static hipError_t launch_rows_ti(const StageArgs<T>& a, hipStream_t st)
     g.tiles_j = (a.M + g.joff + W - 1) / W;
     g.map.ntiles = g.tiles_i * g.tiles_j;
     g.map.per_xcd = (g.map.ntiles + 7) / 8;
+    // FP64 tiles of more than 16 extended rows: 8 waves (half the rows per
+    // wave, twice the waves to hide latency); otherwise 4
+    constexpr int NW = (std::is_same<T, double>::value && RowsCfg<METHOD, FIRST, LAST, TI>::TE > 16) ? 8 : 4;
     if constexpr (std::is_same<T, float>::value && LAST == 4) {
         if (a.ulo) {
-            hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, true>), dim3(8 * g.map.per_xcd), dim3(256),
-                               0, st, a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+            hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, true, NW>), dim3(8 * g.map.per_xcd),
+                               dim3(NW * 64), 0, st, a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, false>), dim3(8 * g.map.per_xcd), dim3(256), 0, st,
-                       a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+    hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, false, NW>), dim3(8 * g.map.per_xcd), dim3(NW * 64),
+                       0, st, a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
     return hipGetLastError();
 }
 

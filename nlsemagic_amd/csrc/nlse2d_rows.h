@@ -1,12 +1,12 @@
 // nlse2d_rows.h -- row-sweep LDS tile for fused 2D RK4 stages (device code).
 //
-// A workgroup of four waves advances a tile of TI rows x W = 64 - 2H owned
+// A workgroup of NW (4 or 8) waves advances a tile of TI rows x W = 64 - 2H owned
 // columns through RK4 stages FIRST..LAST (H = R per fused stage):
 //   - lane l is column j0 - H + l, so one row of the extended tile (the owned
 //     block plus H rows/columns of halo) is exactly one wave-wide access:
 //     a coalesced 1 KB (FP64) load from HBM and conflict-free 16 B per lane
 //     in LDS;
-//   - extended row e belongs to wave e mod 4 in every phase, which keeps the
+//   - extended row e belongs to wave e mod NW in every phase, which keeps the
 //     row's Y, U, V and ktot in that wave's registers; the rows are loaded
 //     all at once at the start (every load in flight together);
 //   - j +- 1 neighbours of a row's own values come from adjacent lanes (DPP
@@ -42,14 +42,14 @@ template <typename T> __device__ __forceinline__ void pin_here(c2<T>& v)
     asm volatile("" : "+v"(v.x), "+v"(v.y));
 }
 
-template <int METHOD, int FIRST, int LAST, int TI>
+template <int METHOD, int FIRST, int LAST, int TI, int NW = 4>
 struct RowsCfg {
     static constexpr int R = (METHOD == 2) ? 2 : 1;
     static constexpr int NF = LAST - FIRST + 1;
     static constexpr int H = NF * R;        // halo rows and columns
     static constexpr int W = 64 - 2 * H;    // owned columns
     static constexpr int TE = TI + 2 * H;   // extended rows
-    static constexpr int RW = (TE + 3) / 4; // extended rows per wave
+    static constexpr int RW = (TE + NW - 1) / NW;  // extended rows per wave
     static constexpr int ND = (METHOD == 2) ? TE - 2 : 1;  // D rows held in LDS (rows 1..TE-2)
 };
 
@@ -76,17 +76,17 @@ __device__ __forceinline__ bool rows_inner(int i0, int j0, int N, int M)
     return i0 - Cfg::H >= 0 && i0 - Cfg::H + Cfg::TE <= N && j0 - Cfg::H >= 1 && j0 - Cfg::H + 64 <= M - 1;
 }
 
-template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP, bool INNER>
+template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP, bool INNER, int NW = 4>
 __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>& sm, int i0, int j0, int N, int M,
                                             const c2<T>* __restrict__ yin, const c2<T>* u, c2<T>* __restrict__ acc,
                                             c2<T>* out, const T* __restrict__ V, const Coefs<T>& c, int BC,
                                             c2<T>* __restrict__ ulo)
 {
-    using Cfg = RowsCfg<METHOD, FIRST, LAST, TI>;
+    using Cfg = RowsCfg<METHOD, FIRST, LAST, TI, NW>;
     constexpr int R = Cfg::R, NF = Cfg::NF, H = Cfg::H, W = Cfg::W, TE = Cfg::TE, RW = Cfg::RW;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    __builtin_assume(w >= 0 && w < 4);
+    __builtin_assume(w >= 0 && w < NW);
     const int j = j0 - H + lane;
     const bool jin = INNER || (j >= 0 && j < M);
     const bool own = jin && lane >= H && lane < H + W;
@@ -121,7 +121,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
     // phase 0: every load of the tile in flight at once
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
-        const int e = 4 * k + w;
+        const int e = NW * k + w;
         const int g = g0 + e;
         const int gc = INNER ? g : (g < 0 ? 0 : (g >= N ? N - 1 : g));
         const long p = (long)gc * M + jc;
@@ -139,7 +139,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
     }
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
-        const int e = 4 * k + w;
+        const int e = NW * k + w;
         if constexpr (FIRST == 1) Uc[k] = Yc[k];
         if (e < TE) sm.y[1 + e * 64 + lane] = Yc[k];
     }
@@ -159,7 +159,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
         if constexpr (METHOD == 2) {
 #pragma unroll
             for (int k = 0; k < RW; ++k) {
-                const int e = 4 * k + w;
+                const int e = NW * k + w;
                 if (e < dlo || e >= dhi) continue;
                 const int g = g0 + e;
                 const c2<T> yx = Yc[k];
@@ -185,7 +185,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
             if (top || bot) {  // MSD rows 0 / N-1 from rows 1 / N-2 (uniform over the workgroup)
 #pragma unroll
                 for (int k = 0; k < RW; ++k) {
-                    const int e = 4 * k + w;
+                    const int e = NW * k + w;
                     if (e < dlo || e >= dhi) continue;
                     const int g = g0 + e;
                     if (g != 0 && g != N - 1) continue;
@@ -237,7 +237,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
         // F, which goes to LDS
 #pragma unroll
         for (int k = 0; k < RW; ++k) {
-            const int e = 4 * k + w;
+            const int e = NW * k + w;
             if (e < flo || e >= fhi) continue;
             const int g = g0 + e;
             const c2<T> yx = Yc[k];
@@ -280,7 +280,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
         if (top || bot) {
 #pragma unroll
             for (int k = 0; k < RW; ++k) {
-                const int e = 4 * k + w;
+                const int e = NW * k + w;
                 if (e < flo || e >= fhi) continue;
                 const int g = g0 + e;
                 if (g == 0) epilogue(k, e, msd_f<T>(colq(sm.y[1 + lane]), colq(ypart[k]), Yc[k]));
@@ -290,7 +290,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
         if (q < NF - 1) {
 #pragma unroll
             for (int k = 0; k < RW; ++k) {
-                const int e = 4 * k + w;
+                const int e = NW * k + w;
                 if (e >= flo && e < fhi) sm.y[1 + e * 64 + lane] = Yc[k];
             }
         }
