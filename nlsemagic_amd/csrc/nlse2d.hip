@@ -125,9 +125,7 @@ static hipError_t launch_rows_ti(const StageArgs<T>& a, hipStream_t st)
     g.tiles_j = (a.M + g.joff + W - 1) / W;
     g.map.ntiles = g.tiles_i * g.tiles_j;
     g.map.per_xcd = (g.map.ntiles + 7) / 8;
-    // FP64 tiles of more than 16 extended rows: 8 waves (half the rows per
-    // wave, twice the waves to hide latency); otherwise 4
-    constexpr int NW = (std::is_same<T, double>::value && RowsCfg<METHOD, FIRST, LAST, TI>::TE > 16) ? 8 : 4;
+    constexpr int NW = rows_waves<METHOD, FIRST, LAST, TI>();
     if constexpr (std::is_same<T, float>::value && LAST == 4) {
         if (a.ulo) {
             hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, true, NW>), dim3(8 * g.map.per_xcd),

@@ -36,8 +36,8 @@ __global__ __launch_bounds__(256) void batch2d_kernel(const c2<T>* __restrict__ 
 }
 
 // the same walk over row-sweep tiles (tile width 64 - 2H per problem)
-template <typename T, int METHOD, int TI, int FIRST, int LAST, bool COMP>
-__global__ __launch_bounds__(256) void batch2d_rows_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+template <typename T, int METHOD, int TI, int FIRST, int LAST, bool COMP, int NW>
+__global__ __launch_bounds__(NW * 64) void batch2d_rows_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
                                                            c2<T>* __restrict__ acc, c2<T>* out,
                                                            const T* __restrict__ V,
                                                            const BatchProb2<T>* __restrict__ probs,
@@ -57,11 +57,12 @@ __global__ __launch_bounds__(256) void batch2d_rows_kernel(const c2<T>* __restri
         const int i0 = ti * TI - P.ioff, j0 = tj * W - P.joff;
         c2<T>* lo = COMP ? ulo + P.off : nullptr;
         if (rows_inner<METHOD, FIRST, LAST, TI>(i0, j0, P.N, P.M))
-            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true>(sm, i0, j0, P.N, P.M, yin + P.off, u + P.off,
-                                                                acc + P.off, out + P.off, V + P.off, P.c, P.BC, lo);
+            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true, NW>(sm, i0, j0, P.N, P.M, yin + P.off, u + P.off,
+                                                                    acc + P.off, out + P.off, V + P.off, P.c, P.BC, lo);
         else
-            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, false>(sm, i0, j0, P.N, P.M, yin + P.off, u + P.off,
-                                                                 acc + P.off, out + P.off, V + P.off, P.c, P.BC, lo);
+            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, false, NW>(sm, i0, j0, P.N, P.M, yin + P.off, u + P.off,
+                                                                     acc + P.off, out + P.off, V + P.off, P.c, P.BC,
+                                                                     lo);
         __syncthreads();  // the next tile reuses the LDS
     }
 }
@@ -69,15 +70,17 @@ __global__ __launch_bounds__(256) void batch2d_rows_kernel(const c2<T>* __restri
 template <typename T, int METHOD, int TI, int FIRST, int LAST>
 static hipError_t launch_b_rows(const BatchArgs<T>& a, hipStream_t st)
 {
+    constexpr int NW = rows_waves<METHOD, FIRST, LAST, TI>();
     if constexpr (std::is_same<T, float>::value && LAST == 4) {
         if (a.ulo) {
-            hipLaunchKernelGGL((batch2d_rows_kernel<T, METHOD, TI, FIRST, LAST, true>), dim3(a.grid), dim3(256), 0,
-                               st, a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.ulo);
+            hipLaunchKernelGGL((batch2d_rows_kernel<T, METHOD, TI, FIRST, LAST, true, NW>), dim3(a.grid),
+                               dim3(NW * 64), 0, st, a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles,
+                               a.ulo);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((batch2d_rows_kernel<T, METHOD, TI, FIRST, LAST, false>), dim3(a.grid), dim3(256), 0, st,
-                       a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.ulo);
+    hipLaunchKernelGGL((batch2d_rows_kernel<T, METHOD, TI, FIRST, LAST, false, NW>), dim3(a.grid), dim3(NW * 64), 0,
+                       st, a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.ulo);
     return hipGetLastError();
 }
 

@@ -43,6 +43,17 @@ template <typename T> __device__ __forceinline__ void pin_here(c2<T>& v)
 }
 
 template <int METHOD, int FIRST, int LAST, int TI, int NW = 4>
+struct RowsCfg;
+
+// waves per workgroup: tiles of more than 16 extended rows use 8 (half the
+// rows per wave, twice the waves to hide latency), smaller ones 4
+template <int METHOD, int FIRST, int LAST, int TI>
+constexpr int rows_waves()
+{
+    return RowsCfg<METHOD, FIRST, LAST, TI>::TE > 16 ? 8 : 4;
+}
+
+template <int METHOD, int FIRST, int LAST, int TI, int NW>
 struct RowsCfg {
     static constexpr int R = (METHOD == 2) ? 2 : 1;
     static constexpr int NF = LAST - FIRST + 1;
