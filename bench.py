@@ -233,6 +233,30 @@ def vortex_rings_3d(n=256, L=25.6, nrings=4, radius=8.0, seed=0):
     return U, h
 
 
+VORT_BYTES = 96  # per point: velocity pass 16 B in + 24 B out, curl pass 24 B in + 32 B out
+
+
+def time_vorticity(sol, h, veps=1e-3, reps=20):
+    """Device vorticity of the resident 3D state (rmc_vorticity3D): wall time
+    per call over `reps` back-to-back calls into device arrays (the two
+    kernels run in a fraction of a millisecond, launch costs are a few us),
+    and the host result of one more call for the parity check."""
+    import nlsemagic_amd as nm
+    n = int(np.prod(sol.shape))
+    outs = [nm.DeviceArray(n) for _ in range(4)]
+    ptrs = [d.address for d in outs]
+    sol.vorticity_device(1 / (2 * h), veps, *ptrs)
+    sol.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sol.vorticity_device(1 / (2 * h), veps, *ptrs)
+    sol.sync()
+    dt = (time.perf_counter() - t0) / reps
+    for d in outs:
+        d.free()
+    return dict(ms=dt * 1e3, rate=n / dt, gbs=VORT_BYTES * n / dt / 1e9, host=sol.vorticity(1 / (2 * h), veps))
+
+
 def run_grid(args):
     """1D (N = 2048, CD, MSD, dark soliton) or 3D (256^3, Dirichlet, vortex
     rings) single-problem workloads: device rate for FP64 / FP32 / FP32c and
@@ -271,6 +295,9 @@ def run_grid(args):
             W = sol.state() if prec == 2 or args.workload == "1d" else None
             out[(m, prec)] = dict(rate=npts * chunk * nchunks / (sum(lat) * 1e-3), chunk_ms=float(np.median(lat)),
                                   mass=mass, bad=bad, W=W)
+            if args.workload == "3d" and prec == 2 and m == methods[-1]:
+                vort = time_vorticity(sol, h)
+                vort["W"] = W
             sol.close()
     # CPU integrator on a bounded sample (1 thread): the reference for 1D, the restatement for 3D
     use_ref = args.workload == "1d" and O.ref(1) is not None
@@ -295,6 +322,17 @@ def run_grid(args):
                    "cpu_rate_1thread": {('CD' if m == 1 else '2SHOC'): cpu[m] for m in methods},
                    "cpu_kind": "reference" if use_ref else "port"},
     }
+    if args.workload == "3d":
+        t0 = time.perf_counter()
+        exp = O.vorticity3d(vort["W"], 1 / (2 * h), 1e-3)
+        cpu_v = npts / (time.perf_counter() - t0)
+        same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(vort["host"], exp))
+        line["config"]["vorticity"] = {
+            "what": "rmc_vorticity3D of the 2SHOC FP64 state after the timed chunks (lh2 = 1/(2h), veps = 1e-3)",
+            "ms_per_call": vort["ms"], "points_per_s": vort["rate"],
+            "algorithmic_GBps": vort["gbs"], "hbm_frac": vort["gbs"] / HBM_PEAK_GBS,
+            "bytes_per_point": VORT_BYTES, "bit_identical_to_oracle": bool(same),
+            "cpu_points_per_s_1thread": cpu_v}
     if args.workload == "1d":
         exp = O.take_steps(U, V, -1.0, 1.0, h * h, BC, chunk * (nchunks + 0), ks[1], 1)
         line["config"]["fp64_bit_identical_to_oracle"] = bool(

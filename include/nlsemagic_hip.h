@@ -80,6 +80,13 @@ int NLSE3D_TAKE_STEPS_HIP_F(const float *Uoldr, const float *Uoldi, const float 
                             float *Unewr, float *Unewi, float s, float a, float h2,
                             int BC, int chunk_size, float k, int N, int M, int L,
                             int method);
+/* Replaces the rmc_vorticity3D MEX post-processor
+ * (src/NLSEmagic3D/rmc_vorticity3D.c:24-354; [wx,wy,wz,W] =
+ * rmc_vorticity3D(U, 1/(2h), veps)): host planes Ur, Ui (Ui may be NULL for
+ * a real U) of M x N x L doubles in MATLAB order (M fastest), outputs of the
+ * same size.  Edge and corner handling: see nlse_vorticity. */
+int NLSE3D_VORTICITY_HIP(const double *Ur, const double *Ui, double lh2, double veps, int M, int N, int L,
+                         double *wx, double *wy, double *wz, double *W);
 
 /* ---- resident-state API ---- */
 typedef struct nlse_solver nlse_solver;
@@ -111,6 +118,22 @@ int nlse_query(nlse_solver *h);
  * out3[0] = sum |U|^2 (mass / h^d), out3[1] = max Re U, out3[2] = number of
  * non-finite points (the reference drivers' blow-up test) */
 int nlse_diagnostics(nlse_solver *h, double *out3);
+/* 3D vorticity of the resident state (replaces the rmc_vorticity3D MEX file,
+ * src/NLSEmagic3D/rmc_vorticity3D.c:24-354, as called by NLSE3D.m:594):
+ * wx, wy, wz = curl of the velocity Im(conj(U) grad U) / (2|U|^2 + veps) and
+ * W = |w|, with lh2 = 1/(2h) and the reference's difference rules, in double
+ * precision whatever the solver's precision (a compensated state is widened
+ * hi + lo first).  Outputs are host arrays of n doubles in the state's
+ * layout; any may be NULL.  The reference leaves the gradient unset on edges
+ * and corners; here it is 0 there, so the result equals the reference's at
+ * every point whose stencil avoids edges and corners, and is 0 on them.
+ * Needs a 3D solver with >= 3 points along every axis; synchronous. */
+int nlse_vorticity(nlse_solver *h, double lh2, double veps, double *wx, double *wy, double *wz, double *W);
+/* the same into device arrays (n doubles each, any may be NULL), enqueued
+ * on the solver's stream without allocating (the velocity is staged in the
+ * solver's RK4 work buffers) */
+int nlse_vorticity_device(nlse_solver *h, double lh2, double veps, double *wx, double *wy, double *wz,
+                          double *W);
 /* run `steps` steps with a HIP event around every kernel launch and return
  * the mean device time of each launch of a step in ms_launch[0..nlaunch-1]
  * (nlaunch = 4 / stages per launch; per-kernel timing for roofline reports;

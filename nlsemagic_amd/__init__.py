@@ -30,7 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnlsemagic_hip.so")
 
 __all__ = [
-    "lib", "Solver", "Batch", "HipError",
+    "lib", "Solver", "Batch", "HipError", "DeviceArray", "rmc_vorticity3D",
     "NLSEmagic1D_Take_Steps_GPU", "NLSEmagic2D_Take_Steps_GPU", "NLSEmagic3D_Take_Steps_GPU",
 ]
 
@@ -126,6 +126,12 @@ def lib():
     L.nlse_diagnostics.argtypes = [C.c_void_p, _D]
     L.nlse_batch_diagnostics.restype = C.c_int
     L.nlse_batch_diagnostics.argtypes = [C.c_void_p, _D]
+    L.NLSE3D_VORTICITY_HIP.restype = C.c_int
+    L.NLSE3D_VORTICITY_HIP.argtypes = [_D, _D, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int] + [_D] * 4
+    L.nlse_vorticity.restype = C.c_int
+    L.nlse_vorticity.argtypes = [C.c_void_p, C.c_double, C.c_double] + [_D] * 4
+    L.nlse_vorticity_device.restype = C.c_int
+    L.nlse_vorticity_device.argtypes = [C.c_void_p, C.c_double, C.c_double] + [C.c_void_p] * 4
     L.nlse_copy_probe.restype = C.c_int
     L.nlse_copy_probe.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.nlse_footprint_bytes.restype = C.c_longlong
@@ -213,6 +219,30 @@ def _v2(dim):
     gateway.__doc__ = (f"v2 gateway U = NLSEmagic{dim}D_Take_Steps_GPU(U,V,s,a,h2,BC,chunk_size,k,precision,method)"
                        " (src/NLSEmagic1D/NLSEmagic1D_Take_Steps_GPU.c:5-90).")
     return gateway
+
+
+def rmc_vorticity3D(U, lh2, veps):
+    """``[wx, wy, wz, W] = rmc_vorticity3D(U, 1/(2h), veps)`` on the GPU
+    (src/NLSEmagic3D/rmc_vorticity3D.c:24-354; called at NLSE3D.m:594,1109).
+
+    U: 3D array of shape (L, N, M) (a MATLAB M x N x L array), real or
+    complex.  Returns four float64 arrays of U's shape: the curl of the
+    velocity Im(conj(U) grad U) / (2|U|^2 + veps) and its norm, evaluated in
+    double precision with the reference's difference rules.  The reference
+    leaves the gradient unset on edges and corners; here it is 0 there, so
+    the result equals the reference's wherever its stencil avoids edges and
+    corners, and edges and corners are 0.
+    """
+    U = np.asarray(U)
+    if U.ndim != 3:
+        raise ValueError(f"expected a 3D array, got shape {U.shape}")
+    L_, N, M = U.shape
+    ur = np.ascontiguousarray(U.real, dtype=np.float64)
+    ui = np.ascontiguousarray(U.imag, dtype=np.float64) if np.iscomplexobj(U) else None
+    out = [np.empty(U.shape) for _ in range(4)]
+    _check(lib().NLSE3D_VORTICITY_HIP(_p(ur, C.c_double), _p(ui, C.c_double) if ui is not None else None,
+                                      float(lh2), float(veps), M, N, L_, *[_p(o, C.c_double) for o in out]))
+    return tuple(out)
 
 
 NLSEmagic1D_Take_Steps_GPU = _v2(1)
@@ -306,6 +336,20 @@ class Solver:
         out = np.empty(3)
         _check(self._lib.nlse_diagnostics(self._h, _p(out, C.c_double)))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def vorticity(self, lh2, veps):
+        """(wx, wy, wz, W) of the resident 3D state, as :func:`rmc_vorticity3D`
+        (double precision whatever the solver's precision)."""
+        out = [np.empty(self.shape) for _ in range(4)]
+        _check(self._lib.nlse_vorticity(self._h, float(lh2), float(veps), *[_p(o, C.c_double) for o in out]))
+        return tuple(out)
+
+    def vorticity_device(self, lh2, veps, wx=0, wy=0, wz=0, W=0):
+        """Enqueue the vorticity into device arrays given as raw addresses
+        (n doubles each, e.g. DeviceArray.address; 0 to skip one) on the
+        solver's stream."""
+        _check(self._lib.nlse_vorticity_device(self._h, float(lh2), float(veps), *[C.c_void_p(int(x) or None)
+                                                                                    for x in (wx, wy, wz, W)]))
 
     def profile_launches(self, steps):
         """Mean device ms of each kernel launch of an RK4 step (HIP events)."""
@@ -451,6 +495,62 @@ class Batch:
         _check(self._lib.nlse_batch_autotune(self._h, int(steps), C.byref(f), C.byref(t), C.byref(g),
                                              C.byref(ms)))
         return f.value, t.value, g.value, ms.value
+
+
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime the library links (already loaded by lib()); a second
+    runtime in the process (torch wheels bundle their own) would not see the
+    GPU once this one has initialised it."""
+    global _hip_rt
+    if _hip_rt is None:
+        lib()
+        H = C.CDLL("libamdhip64.so.7", mode=os.RTLD_NOLOAD | os.RTLD_GLOBAL)
+        H.hipMalloc.restype = C.c_int
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.restype = C.c_int
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.restype = C.c_int
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip_rt = H
+    return _hip_rt
+
+
+class DeviceArray:
+    """A device buffer of `n` elements of `dtype` for the ``*_device`` entry
+    points; ``ptr`` is its address, ``numpy()`` copies it to the host."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.ptr = C.c_void_p()
+        rc = _hip().hipMalloc(C.byref(self.ptr), self.n * self.dtype.itemsize)
+        if rc != 0:
+            raise HipError(f"hipMalloc of {self.n * self.dtype.itemsize} bytes failed ({rc})")
+
+    @property
+    def address(self):
+        return int(self.ptr.value or 0)
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        rc = _hip().hipMemcpy(out.ctypes.data, self.ptr, self.n * self.dtype.itemsize, 2)  # device to host
+        if rc != 0:
+            raise HipError(f"hipMemcpy failed ({rc})")
+        return out
+
+    def free(self):
+        if self.ptr:
+            _hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def copy_probe(nbytes, reps=10):

@@ -20,7 +20,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnlsemagic_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["nlse1d.hip", "nlse2d.hip", "nlse2d_batch.hip", "nlse3d.hip", "nlse_capi.hip", "nlse_batch.hip", "nlse_diag.hip"]
+SOURCES = ["nlse1d.hip", "nlse2d.hip", "nlse2d_batch.hip", "nlse3d.hip", "nlse_capi.hip", "nlse_batch.hip", "nlse_diag.hip",
+           "nlse_vort.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NLSE_OFFLOAD_ARCH", "gfx950")
@@ -49,6 +50,8 @@ MEXDIR = os.path.join(HERE, "mex")
 GATEWAYS = {f"NLSE{d}D_TAKE_STEPS_{m}_CUDA_{p}": [f"-DNLSE_DIM={d}", f"-DNLSE_METHOD={mi}", f"-DNLSE_PREC={pi}"]
             for d in (1, 2, 3) for m, mi in (("CD", 1), ("2SHOC", 2)) for p, pi in (("D", 2), ("F", 1))}
 GATEWAYS.update({f"NLSEmagic{d}D_Take_Steps_GPU": [f"-DNLSE_DIM={d}", "-DNLSE_V2"] for d in (1, 2, 3)})
+# gateways with a source of their own: name -> source under mex/
+OTHER_GATEWAYS = {"rmc_vorticity3D": "nlse_mex_vorticity.c"}
 
 
 def build_test_gateways(mex_include: str, extra_src=(), outdir=None) -> dict:
@@ -59,10 +62,12 @@ def build_test_gateways(mex_include: str, extra_src=(), outdir=None) -> dict:
     os.makedirs(outdir, exist_ok=True)
     inc = os.path.join(os.path.dirname(HERE), "include")
     out = {}
-    for name, defs in GATEWAYS.items():
+    jobs = [(name, defs, "nlse_mex_gateway.c") for name, defs in GATEWAYS.items()]
+    jobs += [(name, [], src) for name, src in OTHER_GATEWAYS.items()]
+    for name, defs, src in jobs:
         so = os.path.join(outdir, name + ".so")
         cmd = ["gcc", "-std=c99", "-O2", "-fPIC", "-shared", "-Wall", "-Werror", *defs, f"-I{mex_include}",
-               f"-I{inc}", os.path.join(MEXDIR, "nlse_mex_gateway.c"), *extra_src, f"-L{HERE}", "-lnlsemagic_hip",
+               f"-I{inc}", os.path.join(MEXDIR, src), *extra_src, f"-L{HERE}", "-lnlsemagic_hip",
                f"-Wl,-rpath,{HERE}", "-o", so]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:

@@ -564,6 +564,45 @@ int nlse_diagnostics(nlse_solver* h, double* out3)
     return NLSE_OK;
 }
 
+int nlse_vorticity_device(nlse_solver* h, double lh2, double veps, double* wx, double* wy, double* wz, double* W)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    if (h->dim != 3) return fail(NLSE_EINVAL, "vorticity is defined for 3D solvers");
+    if (h->N < 3 || h->M < 3 || h->L < 3) return fail(NLSE_EINVAL, "vorticity needs >= 3 points along every axis");
+    // the velocity goes into the RK4 work buffers Y0, Y1, ktot (>= 8 bytes per
+    // point each), free between steps: a 3D solver's state stays in U
+    double *vx = (double*)h->Y0, *vy = (double*)h->Y1, *vz = (double*)h->acc;
+    hipError_t e = is_double(h)
+        ? launch_vorticity3d<double>((const double2*)h->U, nullptr, h->L, h->N, h->M, lh2, veps, vx, vy, vz, wx, wy,
+                                     wz, W, h->stream)
+        : launch_vorticity3d<float>((const float2*)h->U, (const float2*)h->Ulo, h->L, h->N, h->M, lh2, veps, vx, vy,
+                                    vz, wx, wy, wz, W, h->stream);
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("vorticity: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
+
+int nlse_vorticity(nlse_solver* h, double lh2, double veps, double* wx, double* wy, double* wz, double* W)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    double* host[4] = {wx, wy, wz, W};
+    double* dev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* buf = nullptr;
+    const size_t b = sizeof(double) * (size_t)h->n;
+    int nout = 0;
+    for (double* p : host) nout += p != nullptr;
+    if (nout && hipMalloc((void**)&buf, b * nout) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    for (int o = 0, q = 0; o < 4; ++o)
+        if (host[o]) dev[o] = buf + (size_t)h->n * q++;
+    int rc = nlse_vorticity_device(h, lh2, veps, dev[0], dev[1], dev[2], dev[3]);
+    hipError_t e = hipSuccess;
+    for (int o = 0; o < 4 && rc == NLSE_OK && e == hipSuccess; ++o)
+        if (host[o]) e = hipMemcpyAsync(host[o], dev[o], b, hipMemcpyDeviceToHost, h->stream);
+    if (rc == NLSE_OK && e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (buf) (void)hipFree(buf);
+    if (rc == NLSE_OK && e != hipSuccess) rc = fail(NLSE_EHIP, std::string("vorticity: ") + hipGetErrorString(e));
+    return rc;
+}
+
 int nlse_query(nlse_solver* h)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");
@@ -763,6 +802,42 @@ int NLSE3D_TAKE_STEPS_HIP_F(const float* Uoldr, const float* Uoldi, const float*
                             int method)
 {
     return take_steps<float>(3, Uoldr, Uoldi, V, Unewr, Unewi, s, a, h2, BC, chunk_size, k, N, M, L, method);
+}
+
+int NLSE3D_VORTICITY_HIP(const double* Ur, const double* Ui, double lh2, double veps, int M, int N, int L,
+                         double* wx, double* wy, double* wz, double* W)
+{
+    if (!Ur || !wx || !wy || !wz || !W) return fail(NLSE_EINVAL, "null array argument");
+    if (N < 3 || M < 3 || L < 3) return fail(NLSE_EINVAL, "vorticity needs >= 3 points along every axis");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NLSE_EHIP, "no HIP device available");
+    const long n = (long)N * M * L;
+    const size_t b = sizeof(double) * (size_t)n;
+    // U (2n), velocity (3n, also the upload staging area), wx, wy, wz, W (4n)
+    double* buf = nullptr;
+    if (hipMalloc((void**)&buf, b * 9) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipFree(buf);
+        return fail(NLSE_EHIP, "stream creation failed");
+    }
+    double2* u = (double2*)buf;
+    double* v = buf + 2 * n;
+    double* w = buf + 5 * n;
+    int rc = upload_planes<double>(Ur, Ui, u, v, n, st);
+    hipError_t e = hipSuccess;
+    if (rc == NLSE_OK)
+        e = launch_vorticity3d<double>(u, nullptr, L, N, M, lh2, veps, v, v + n, v + 2 * n, w, w + n, w + 2 * n,
+                                       w + 3 * n, st);
+    double* host[4] = {wx, wy, wz, W};
+    for (int o = 0; o < 4 && rc == NLSE_OK && e == hipSuccess; ++o)
+        e = hipMemcpyAsync(host[o], w + (size_t)o * n, b, hipMemcpyDeviceToHost, st);
+    if (rc == NLSE_OK && e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    (void)hipFree(buf);
+    if (rc == NLSE_OK && e != hipSuccess) rc = fail(NLSE_EHIP, std::string("vorticity: ") + hipGetErrorString(e));
+    return rc;
 }
 
 }  // extern "C"

@@ -75,6 +75,15 @@ void oracle_rhs3d(const double *Ur, const double *Ui, const double *V,
 /* Diagnostics on a state: sum |U|^2 in double (mass without the h^d factor) */
 double oracle_sum_abs2(const double *Ur, const double *Ui, long n);
 
+/* 3D vorticity of a state (src/NLSEmagic3D/rmc_vorticity3D.c:24-354):
+ * wx, wy, wz = curl of V = Im(conj(U) grad U) / (2|U|^2 + veps), W = |w|;
+ * lh2 = 1/(2h); dims M x N x L as oracle_nlse3d (nlse_vorticity.c). */
+void oracle_vorticity3d(const double *Ur, const double *Ui, double lh2, double veps, int M, int N, int L,
+                        double *wx, double *wy, double *wz, double *W);
+/* the same with the unset edge/corner gradient given the value edge_g */
+void oracle_vorticity3d_fill(const double *Ur, const double *Ui, double lh2, double veps, int M, int N, int L,
+                             double edge_g, double *wx, double *wy, double *wz, double *W);
+
 #ifdef __cplusplus
 }
 #endif

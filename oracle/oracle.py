@@ -54,6 +54,11 @@ def lib():
             fr.argtypes = [_D] * 5 + [C.c_double] * 3 + [C.c_int] * (extra + 2)
         _lib.oracle_sum_abs2.restype = C.c_double
         _lib.oracle_sum_abs2.argtypes = [_D, _D, C.c_long]
+        _lib.oracle_vorticity3d.restype = None
+        _lib.oracle_vorticity3d.argtypes = [_D, _D, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int] + [_D] * 4
+        _lib.oracle_vorticity3d_fill.restype = None
+        _lib.oracle_vorticity3d_fill.argtypes = ([_D, _D, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                                  C.c_double] + [_D] * 4)
     return _lib
 
 
@@ -154,3 +159,43 @@ def rhs(U, V, s, a, h2, BC, method):
         _p(fi, C.c_double), float(s), float(a), float(h2), int(BC),
         *[int(t) for t in _dims(U.shape)], int(method))
     return _cplx(fr, fi)
+
+
+def vorticity3d(U, lh2, veps, edge_g=None):
+    """3D vorticity of a state (rmc_vorticity3D.c:24-354): returns
+    (wx, wy, wz, W) as arrays of U's shape (L, N, M); lh2 = 1/(2h).  The
+    gradient the reference leaves unset on edges and corners is 0, or
+    edge_g when given."""
+    U = np.asarray(U)
+    assert U.ndim == 3
+    L, N, M = U.shape
+    ur = np.ascontiguousarray(U.real, dtype=np.float64)
+    ui = np.ascontiguousarray(U.imag, dtype=np.float64)
+    out = [np.empty(U.shape) for _ in range(4)]
+    args = (_p(ur, C.c_double), _p(ui, C.c_double), float(lh2), float(veps), M, N, L)
+    if edge_g is None:
+        lib().oracle_vorticity3d(*args, *[_p(o, C.c_double) for o in out])
+    else:
+        lib().oracle_vorticity3d_fill(*args, float(edge_g), *[_p(o, C.c_double) for o in out])
+    return tuple(out)
+
+
+def vorticity_defined_mask(shape):
+    """Points where the reference's vorticity is defined.  The reference
+    leaves the gradient, and so the velocity, unset on edges and corners
+    (rmc_vorticity3D.c:73-86,113-184); the curl at a point is defined when
+    none of the velocities its stencil reads sits on an edge or corner:
+    every interior point; on an x-face (i = 0 or L-1) the points with j and
+    k at least 2 from the boundary (central y and z differences); on a y- or
+    z-face the points whose central tangential index is at least 2 from the
+    boundary and whose forward x-difference does not reach i = L-1
+    (1 <= i <= L-3).  Edges and corners themselves are never defined."""
+    L, N, M = shape
+    i, j, k = np.meshgrid(np.arange(L), np.arange(N), np.arange(M), indexing="ij")
+    bi, bj, bk = (i == 0) | (i == L - 1), (j == 0) | (j == N - 1), (k == 0) | (k == M - 1)
+    nbound = bi.astype(int) + bj + bk
+    far = lambda a, n: (a >= 2) & (a <= n - 3)
+    xface = bi & (nbound == 1) & far(j, N) & far(k, M)
+    yface = bj & (nbound == 1) & far(k, M) & (i <= L - 3)
+    zface = bk & (nbound == 1) & far(j, N) & (i <= L - 3)
+    return (nbound == 0) | xface | yface | zface

@@ -32,5 +32,6 @@ double *mxGetPi(const mxArray *a);
 double mxGetScalar(const mxArray *a);
 mxArray *mxCreateDoubleMatrix(mwSize m, mwSize n, mxComplexity c);
 mxArray *mxCreateNumericArray(mwSize nd, const mwSize *dims, mxClassID cls, mxComplexity c);
+void mxDestroyArray(mxArray *a);
 void mexErrMsgTxt(const char *msg);
 #endif

@@ -64,6 +64,7 @@ static void release(mxArray *a)
     free(a->pi);
     free(a);
 }
+void mxDestroyArray(mxArray *a) { release(a); }
 
 /* returns 0 and fills outr/outi (numel of U), or 1 and the error text */
 int shim_call(int nd, const size_t *dims, const double *ur, const double *ui, const double *v,
@@ -96,5 +97,38 @@ int shim_call(int nd, const size_t *dims, const double *ur, const double *ui, co
     }
     for (int i = 0; i < 2 + nscalars; ++i) release(args[i]);
     release(out[0]);
+    return rc;
+}
+
+/* a gateway of real outputs: U (ur, ui or NULL), then nscalars scalars;
+ * max(nlhs, 1) outputs of U's numel (MATLAB's ans when nlhs = 0) copied to
+ * outs[]; 0 or 1 and the error */
+int shim_call_real(int nd, const size_t *dims, const double *ur, const double *ui, const double *scalars,
+                   int nscalars, int nlhs, double **outs, char *err, int errlen)
+{
+    mxArray *args[16] = {0};
+    mxArray *out[8] = {0};
+    int rc = 0;
+    mwSize d[3];
+    for (int i = 0; i < nd; ++i) d[i] = dims[i];
+    args[0] = make(nd, d, ui != NULL);
+    size_t n = mxGetNumberOfElements(args[0]);
+    memcpy(args[0]->pr, ur, n * sizeof(double));
+    if (ui) memcpy(args[0]->pi, ui, n * sizeof(double));
+    for (int i = 0; i < nscalars; ++i) {
+        mwSize one[2] = {1, 1};
+        args[1 + i] = make(2, one, 0);
+        args[1 + i]->pr[0] = scalars[i];
+    }
+    if (setjmp(g_jmp) == 0) {
+        mexFunction(nlhs, out, 1 + nscalars, (const mxArray **)args);
+        for (int o = 0; o < (nlhs < 1 ? 1 : nlhs); ++o) memcpy(outs[o], out[o]->pr, n * sizeof(double));
+    } else {
+        strncpy(err, g_msg, (size_t)errlen - 1);
+        err[errlen - 1] = 0;
+        rc = 1;
+    }
+    for (int i = 0; i < 1 + nscalars; ++i) release(args[i]);
+    for (int o = 0; o < 8; ++o) release(out[o]);
     return rc;
 }

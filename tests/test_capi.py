@@ -68,3 +68,19 @@ def test_missing_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(nm, "LIB_PATH", "/nonexistent/libnlsemagic_hip.so")
     with pytest.raises(ImportError):
         nm.lib()
+
+
+def test_vorticity_argument_checks_need_no_gpu():
+    import nlsemagic_amd as nm
+    import numpy as np
+    L = nm.lib()
+    a = np.zeros(2 * 3 * 3)
+    p = a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rc = L.NLSE3D_VORTICITY_HIP(p, p, 0.5, 1e-3, 2, 3, 3, p, p, p, p)
+    assert rc == 1 and b">= 3 points" in L.nlse_last_error()
+    rc = L.NLSE3D_VORTICITY_HIP(p, p, 0.5, 1e-3, 3, 3, 3, None, p, p, p)
+    assert rc == 1 and b"null" in L.nlse_last_error()
+    rc = L.nlse_vorticity(None, 0.5, 1e-3, p, p, p, p)
+    assert rc == 1 and b"null solver" in L.nlse_last_error()
+    with pytest.raises(ValueError):
+        nm.rmc_vorticity3D(np.zeros((4, 4)), 0.5, 1e-3)

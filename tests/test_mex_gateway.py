@@ -19,7 +19,7 @@ _D = C.POINTER(C.c_double)
 @pytest.fixture(scope="module")
 def gateways():
     from nlsemagic_amd import build as B
-    out = {n: os.path.join(B.MEXDIR, "_build", n + ".so") for n in B.GATEWAYS}
+    out = {n: os.path.join(B.MEXDIR, "_build", n + ".so") for n in [*B.GATEWAYS, *B.OTHER_GATEWAYS]}
     if not all(os.path.exists(p) for p in out.values()):
         shim = os.path.join(ROOT, "tests", "mex_shim")
         out = B.build_test_gateways(shim, extra_src=[os.path.join(shim, "mex_shim.c")])
@@ -29,6 +29,9 @@ def gateways():
         L.shim_call.restype = C.c_int
         L.shim_call.argtypes = [C.c_int, C.POINTER(C.c_size_t), _D, _D, _D, _D, C.c_int, _D, _D, C.c_char_p,
                                 C.c_int]
+        L.shim_call_real.restype = C.c_int
+        L.shim_call_real.argtypes = [C.c_int, C.POINTER(C.c_size_t), _D, _D, _D, C.c_int, C.c_int,
+                                     C.POINTER(_D), C.c_char_p, C.c_int]
         libs[n] = L
     return libs
 
@@ -56,8 +59,30 @@ def call(lib, U, V, *scalars, matlab_dims=None):
     return out
 
 
+def call_real(lib, U, *scalars, nlhs=4, matlab_dims=None):
+    """A gateway of real outputs (rmc_vorticity3D): U in this project's
+    layout, passed with MATLAB dims (M, N, L); returns nlhs arrays."""
+    U = np.asarray(U)
+    if matlab_dims is None:
+        matlab_dims = tuple(reversed(U.shape))
+    dims = (C.c_size_t * len(matlab_dims))(*matlab_dims)
+    ur = np.ascontiguousarray(U.real, dtype=np.float64)
+    ui = np.ascontiguousarray(U.imag, dtype=np.float64) if np.iscomplexobj(U) else None
+    sc = np.asarray(scalars, dtype=np.float64)
+    outs = [np.empty(U.shape) for _ in range(max(nlhs, 1))]
+    ptrs = (_D * len(outs))(*[o.ctypes.data_as(_D) for o in outs])
+    err = C.create_string_buffer(512)
+    rc = lib.shim_call_real(len(matlab_dims), dims, ur.ctypes.data_as(_D),
+                            ui.ctypes.data_as(_D) if ui is not None else None, sc.ctypes.data_as(_D), len(sc),
+                            nlhs, ptrs, err, 512)
+    if rc:
+        raise RuntimeError(err.value.decode())
+    return outs
+
+
 def test_every_name_is_built(gateways):
     names = set(gateways)
+    assert "rmc_vorticity3D" in names
     for d in (1, 2, 3):
         assert f"NLSEmagic{d}D_Take_Steps_GPU" in names
         for m in ("CD", "2SHOC"):
@@ -103,3 +128,29 @@ def test_real_input_through_the_gateway(gateways, oracle):
     V = np.zeros((16, 24))
     got = call(gateways["NLSE2D_TAKE_STEPS_CD_CUDA_D"], U, V, -1.0, 1.0, 0.01, 1, 4, 0.002)
     assert bits_equal(got, oracle.take_steps(U + 0j, V, -1.0, 1.0, 0.01, 1, 4, 0.002, 1))
+
+
+def test_vorticity_gateway_argument_checks_need_no_gpu(gateways):
+    g = gateways["rmc_vorticity3D"]
+    U = np.ones((4, 5, 6), complex)
+    with pytest.raises(RuntimeError, match="expected 3 arguments"):
+        call_real(g, U, 0.5)
+    with pytest.raises(RuntimeError, match="at most 4 outputs"):
+        call_real(g, U, 0.5, 1e-3, nlhs=5)
+    with pytest.raises(RuntimeError, match="3D array"):
+        call_real(g, np.ones((5, 6), complex), 0.5, 1e-3)
+    with pytest.raises(RuntimeError, match=">= 3 points"):
+        call_real(g, np.ones((4, 5, 2), complex), 0.5, 1e-3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nlhs", [4, 2, 0])
+def test_vorticity_gateway_matches_the_oracle(gateways, oracle, nlhs):
+    rng = np.random.default_rng(21)
+    U = 1 + 0.3 * rng.standard_normal((7, 9, 12)) + 0.3j * rng.standard_normal((7, 9, 12))
+    print("CASE rmc_vorticity3D", nlhs, flush=True)
+    got = call_real(gateways["rmc_vorticity3D"], U, 1 / (2 * 0.25), 1e-3, nlhs=nlhs)
+    exp = oracle.vorticity3d(U, 1 / (2 * 0.25), 1e-3)
+    assert len(got) == max(nlhs, 1)  # nlhs = 0: MATLAB's ans, the first output
+    for g_, e in zip(got, exp):
+        assert bits_equal(g_ + 0j, e + 0j)

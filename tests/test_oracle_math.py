@@ -270,3 +270,100 @@ def test_single_precision_tracks_double(oracle):
             d = oracle.take_steps(U, V, -1.0, 1.0, 0.04, 2, 10, 0.004, method)
             f = oracle.take_steps(U, V, -1.0, 1.0, 0.04, 2, 10, 0.004, method, precision=1)
             assert rel_l2(f, d) < 1e-5
+
+
+# ---- 3D vorticity (rmc_vorticity3D.c:24-354) ------------------------------
+# The reference's vorticity MEX file needs MATLAB's mex.h and is not built
+# here; its restatement (oracle/nlse_vorticity.c) is pinned by what the
+# mathematics fixes: a plane wave carries no vorticity, a straight vortex line
+# of winding +-1 carries circulation +-pi in the reference's normalisation
+# (V = Im(conj U grad U) / (2|U|^2)), axis permutations permute the outputs,
+# and the points where the reference's unset edge gradient leaks in are
+# exactly oracle.vorticity_defined_mask.
+
+def _grid3(shape, h):
+    L, N, M = shape
+    return np.meshgrid(*[(np.arange(n) - (n - 1) / 2) * h for n in (L, N, M)], indexing="ij")
+
+
+def test_vorticity_of_a_plane_wave_vanishes(oracle):
+    h = 0.3
+    x, y, z = _grid3((12, 14, 16), h)
+    U = np.exp(1j * (0.7 * x + 0.4 * y - 0.3 * z))
+    wx, wy, wz, W = oracle.vorticity3d(U, 1 / (2 * h), 1e-3)
+    mask = oracle.vorticity_defined_mask(U.shape)
+    # the face rules keep V exact: Im(conj(U0)(U1 - U0)) = Im(conj(U0) U1) for |U0| = 1
+    assert np.max(W[mask]) < 1e-13
+    # velocity scale, for comparison: |k| / 2 ~ 0.4
+    _, _, _, W2 = oracle.vorticity3d(U * np.exp(0.05j * x ** 2), 1 / (2 * h), 1e-3)
+    assert np.max(W2[mask]) > 1e-3  # a curved phase front is not irrotational in this check's eyes
+
+
+@pytest.mark.parametrize("sign", [1, -1])
+@pytest.mark.parametrize("n", [40, 64])
+def test_vortex_line_carries_circulation_pi(oracle, sign, n):
+    h = 8.0 / n
+    L = 8
+    x, y, z = _grid3((L, n, n), h)
+    r = np.hypot(y, z)
+    U = np.tanh(r / 0.5) * np.exp(sign * 1j * np.arctan2(z, y))
+    wx, wy, wz, W = oracle.vorticity3d(U, 1 / (2 * h), 1e-3)
+    # over indices 2 .. n-3 of every slice, the x-faces included (whose wx
+    # uses central y, z differences; the rows next to the edges read the
+    # unset edge velocity and are left out)
+    circ = wx[:, 2:-2, 2:-2].sum(axis=(1, 2)) * h * h
+    np.testing.assert_allclose(circ, sign * np.pi, rtol=2e-3)
+    # the line runs along the slowest axis: no other component where the
+    # reference is defined
+    mask = oracle.vorticity_defined_mask(U.shape)
+    assert np.max(np.abs(wy[mask])) < 1e-12 and np.max(np.abs(wz[mask])) < 1e-12
+    np.testing.assert_allclose(W[mask], np.abs(wx[mask]), rtol=1e-12, atol=1e-12)
+
+
+def test_vorticity_axis_permutation(oracle):
+    """U'(j, k, i) = U(i, j, k): (wx', wy', wz') = (wy, wz, wx) bit for bit
+    where every difference is central (indices 2 .. n-3)."""
+    rng = np.random.default_rng(5)
+    U = 1 + 0.3 * rng.standard_normal((9, 10, 11)) + 0.3j * rng.standard_normal((9, 10, 11))
+    lh2, veps = 1.7, 1e-3
+    wx, wy, wz, W = oracle.vorticity3d(U, lh2, veps)
+    P = np.ascontiguousarray(U.transpose(1, 2, 0))
+    px, py, pz, PW = [np.ascontiguousarray(a.transpose(2, 0, 1)) for a in oracle.vorticity3d(P, lh2, veps)]
+    core = (slice(2, -2),) * 3
+    np.testing.assert_array_equal(px[core], wy[core])
+    np.testing.assert_array_equal(py[core], wz[core])
+    np.testing.assert_array_equal(pz[core], wx[core])
+    np.testing.assert_allclose(PW[core], W[core], rtol=1e-15)
+
+
+def test_vorticity_is_phase_and_scale_invariant(oracle):
+    """V depends on U only through grad(phase) when veps = 0: a global phase
+    and a global scale leave the vorticity unchanged up to rounding."""
+    rng = np.random.default_rng(6)
+    U = 1 + 0.3 * rng.standard_normal((7, 8, 9)) + 0.3j * rng.standard_normal((7, 8, 9))
+    base = oracle.vorticity3d(U, 1.3, 0.0)
+    for f in (np.exp(0.9j), 3.5, -2j):
+        got = oracle.vorticity3d(U * f, 1.3, 0.0)
+        for a, b in zip(got, base):
+            np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (4, 5, 6), (5, 6, 7), (9, 12, 8)])
+def test_vorticity_defined_mask_is_where_unset_values_leak(oracle, shape):
+    """Fill the gradient the reference leaves unset with NaN: the points that
+    stay finite (edges and corners aside, which the reference never writes)
+    are exactly the defined mask, and there the values do not depend on the
+    fill."""
+    rng = np.random.default_rng(7)
+    U = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    zero = oracle.vorticity3d(U, 0.7, 1e-3)
+    nan = oracle.vorticity3d(U, 0.7, 1e-3, edge_g=np.nan)
+    L, N, M = shape
+    i, j, k = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    nb = ((i == 0) | (i == L - 1)).astype(int) + ((j == 0) | (j == N - 1)) + ((k == 0) | (k == M - 1))
+    edge = nb >= 2
+    mask = oracle.vorticity_defined_mask(shape)
+    np.testing.assert_array_equal(np.isfinite(nan[3]) & ~edge, mask)
+    for a, b in zip(zero, nan):
+        np.testing.assert_array_equal(a[mask], b[mask])
+        assert np.all(a[edge] == 0) and np.all(b[edge] == 0)
