@@ -134,6 +134,63 @@ int nlse_vorticity(nlse_solver *h, double lh2, double veps, double *wx, double *
  * solver's RK4 work buffers) */
 int nlse_vorticity_device(nlse_solver *h, double lh2, double veps, double *wx, double *wy, double *wz,
                           double *W);
+/* ---- closed-form fields and initial conditions of the resident state ----
+ * Grid coordinates follow the reference drivers' meshgrid: 1D x = xmin + i h;
+ * 2D U[i*M + j] at x = xmin + i h, y = ymin + j h; 3D U[(i*N + j)*M + k] at
+ * z = zmin + i h, x = xmin + j h, y = ymin + k h.  Evaluated on the GPU in
+ * double precision (single-precision states are rounded on store; a
+ * compensated state keeps hi + lo); synchronous.
+ *
+ * Exact solutions, kind = the reference's IC code
+ * (src/NLSEmagic1D/NLSEmagic1D_FRS.m:201-225, 535-545; 2D/3D linear case
+ * NLSE2D.m:1010, NLSE3D.m:1272), par = {a, s, OM, c, x0}:
+ *   1 bright soliton sqrt(2 OM/s) sech(sqrt(OM/a)(x - x0 - c t)) e^{i((c/2a)x + (OM - c^2/4a)t)} (1D)
+ *   2 dark soliton   sqrt(OM/s) tanh(sqrt(|OM/2a|)(x - x0 - c t)) e^{same phase}                   (1D)
+ *   3 linear ground state e^{-|x|^2/2a} e^{-i d t} in d dimensions, with V = |x|^2/a
+ *   4 grey soliton (vc = sqrt(-2 a OM), E = sqrt(a/-OM))                                          (1D)
+ * nlse_set_exact writes the solution at time t into the state (kind 3 also
+ * sets V; the others leave V).  nlse_exact_error compares the state with it:
+ * out6 = RMS of Re, Im and |U|^2 differences (NLSE2D.m:1011-1013) followed by
+ * their maxima (NLSEmagic1D_FRS.m:591-593; NaNs skipped as MATLAB's max). */
+int nlse_set_exact(nlse_solver *h, int kind, const double *par, double t, double xmin, double ymin,
+                   double zmin, double hgrid);
+int nlse_exact_error(nlse_solver *h, int kind, const double *par, double t, double xmin, double ymin,
+                     double zmin, double hgrid, double *out6);
+/* Vortex initial conditions combined into the state as the reference's
+ * generators do (s < 0: the normalised fields are multiplied and rescaled to
+ * the new vortex's amplitude; s > 0: the new field is added).  The radial
+ * profile solves the reference's discrete steady state
+ * (NLSEmagic3D/NLS_STEADY_F.m:10-55) by Newton's method with the exact
+ * tridiagonal Jacobian (the reference uses Newton-Krylov, nsoli.m, to the
+ * same fixed point).
+ * Replaces NLS_add_vortex(charge,om,x0,y0,vx,vy,phase,U,h,a,s,amp,X,Y,opt)
+ * (src/NLSEmagic2D/NLSEmagic2D_Full_Research_Scripts/NLS_add_vortex.m:9-158);
+ * a 2D solver. */
+int nlse_add_vortex_2d(nlse_solver *h, int charge, double om, double x0, double y0, double vx, double vy,
+                       double phase, double hgrid, double a, double s, double amp, double xmin, double ymin,
+                       int opt);
+/* Replaces NLSE_add_vortex_line(m,OM,x0,y0,z0,vx,vy,vz,phase,U,h,a,s,amp,X,Y,Z)
+ * (src/NLSEmagic3D/NLSE_add_vortex_line.m:10-125; amp is unused there); a
+ * 3D solver, the line runs along z through (x0, y0). */
+int nlse_add_vortex_line(nlse_solver *h, int m, double OM, double x0, double y0, double z0, double vx,
+                         double vy, double vz, double phase, double hgrid, double a, double s, double xmin,
+                         double ymin, double zmin);
+/* Replaces NLSE_add_vortex_ring(m,OM,x0,y0,z0,thetax,thetay,thetaz,vx,vy,vz,phase,U,h,a,s,X,Y,Z,d,opt_2D,ea,eb)
+ * (src/NLSEmagic3D/NLSE_add_vortex_ring.m:10-534) for opt_2D = 0: the
+ * profile on the (r, z) slice, the ring on the rotated lattice with
+ * ellipticity ea, eb and velocity (vx, vy, vz), quarter turns by axis
+ * permutation and other orientations by the reference's nearest-neighbour
+ * lookup.  opt_2D > 0 (a Newton-Krylov refinement of the slice) is rejected. */
+int nlse_add_vortex_ring(nlse_solver *h, int m, double OM, double x0, double y0, double z0, double thetax,
+                         double thetay, double thetaz, double vx, double vy, double vz, double phase,
+                         double hgrid, double a, double s, double xmin, double ymin, double zmin, double d,
+                         int opt_2D, double ea, double eb);
+/* the radial profile alone (host): f[q] at r = q*hr, q < n, f[0] = 0 and
+ * f[n-1] the far-field value fixed; ansatz 0 dark sqrt(OM/s + a m^2/(s r^2)),
+ * 1 dark sqrt(OM/s) tanh(sqrt(|OM/2a|) r), 2 bright B sech(C (r - R0));
+ * optimize = 1 runs Newton to a residual of tol (1 + |F(ansatz)|) */
+int nlse_vortex_profile(int m, double OM, double a, double s, double hr, int n, int ansatz, int optimize,
+                        double tol, double *f);
 /* run `steps` steps with a HIP event around every kernel launch and return
  * the mean device time of each launch of a step in ms_launch[0..nlaunch-1]
  * (nlaunch = 4 / stages per launch; per-kernel timing for roofline reports;

@@ -30,7 +30,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnlsemagic_hip.so")
 
 __all__ = [
-    "lib", "Solver", "Batch", "HipError", "DeviceArray", "rmc_vorticity3D",
+    "lib", "Solver", "Batch", "HipError", "DeviceArray", "rmc_vorticity3D", "vortex_profile",
+    "NLS_add_vortex", "NLSE_add_vortex_line", "NLSE_add_vortex_ring",
     "NLSEmagic1D_Take_Steps_GPU", "NLSEmagic2D_Take_Steps_GPU", "NLSEmagic3D_Take_Steps_GPU",
 ]
 
@@ -132,6 +133,18 @@ def lib():
     L.nlse_vorticity.argtypes = [C.c_void_p, C.c_double, C.c_double] + [_D] * 4
     L.nlse_vorticity_device.restype = C.c_int
     L.nlse_vorticity_device.argtypes = [C.c_void_p, C.c_double, C.c_double] + [C.c_void_p] * 4
+    L.nlse_set_exact.restype = C.c_int
+    L.nlse_set_exact.argtypes = [C.c_void_p, C.c_int, _D] + [C.c_double] * 5
+    L.nlse_exact_error.restype = C.c_int
+    L.nlse_exact_error.argtypes = [C.c_void_p, C.c_int, _D] + [C.c_double] * 5 + [_D]
+    L.nlse_add_vortex_2d.restype = C.c_int
+    L.nlse_add_vortex_2d.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 12 + [C.c_int]
+    L.nlse_add_vortex_line.restype = C.c_int
+    L.nlse_add_vortex_line.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 14
+    L.nlse_add_vortex_ring.restype = C.c_int
+    L.nlse_add_vortex_ring.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 18 + [C.c_int, C.c_double, C.c_double]
+    L.nlse_vortex_profile.restype = C.c_int
+    L.nlse_vortex_profile.argtypes = ([C.c_int] + [C.c_double] * 4 + [C.c_int] * 3 + [C.c_double, _D])
     L.nlse_copy_probe.restype = C.c_int
     L.nlse_copy_probe.argtypes = [C.c_longlong, C.c_int, C.POINTER(C.c_double)]
     L.nlse_footprint_bytes.restype = C.c_longlong
@@ -351,6 +364,44 @@ class Solver:
         _check(self._lib.nlse_vorticity_device(self._h, float(lh2), float(veps), *[C.c_void_p(int(x) or None)
                                                                                     for x in (wx, wy, wz, W)]))
 
+    # ---- closed-form fields and vortex initial conditions (grid origin
+    # (xmin, ymin, zmin) and spacing h; see include/nlsemagic_hip.h)
+    def set_exact(self, kind, h, origin, t=0.0, a=1.0, s=0.0, OM=0.0, c=0.0, x0=0.0):
+        """Write the exact solution `kind` (1 bright, 2 dark, 3 linear ground
+        state, 4 grey; NLSEmagic1D_FRS.m:201-225) at time t into the state."""
+        par = np.array([a, s, OM, c, x0], dtype=np.float64)
+        o = tuple(origin) + (0.0,) * (3 - len(tuple(origin)))
+        _check(self._lib.nlse_set_exact(self._h, int(kind), _p(par, C.c_double), float(t), *map(float, o), float(h)))
+
+    def exact_error(self, kind, h, origin, t, a=1.0, s=0.0, OM=0.0, c=0.0, x0=0.0):
+        """(rms_re, rms_im, rms_mod2, max_re, max_im, max_mod2) of the state
+        against the exact solution at time t (NLSE2D.m:1011-1013,
+        NLSEmagic1D_FRS.m:591-593)."""
+        par = np.array([a, s, OM, c, x0], dtype=np.float64)
+        o = tuple(origin) + (0.0,) * (3 - len(tuple(origin)))
+        out = np.empty(6)
+        _check(self._lib.nlse_exact_error(self._h, int(kind), _p(par, C.c_double), float(t), *map(float, o), float(h),
+                                          _p(out, C.c_double)))
+        return tuple(float(x) for x in out)
+
+    def add_vortex_2d(self, charge, om, x0, y0, vx, vy, phase, h, a, s, amp, origin, opt=1):
+        """NLS_add_vortex (NLS_add_vortex.m:9-158) on the resident 2D state."""
+        _check(self._lib.nlse_add_vortex_2d(self._h, int(charge), *map(float, (om, x0, y0, vx, vy, phase, h, a, s, amp,
+                                                                             origin[0], origin[1])), int(opt)))
+
+    def add_vortex_line(self, m, OM, x0, y0, z0, vx, vy, vz, phase, h, a, s, origin):
+        """NLSE_add_vortex_line (NLSE_add_vortex_line.m:10-125) on the resident 3D state."""
+        _check(self._lib.nlse_add_vortex_line(self._h, int(m), *map(float, (OM, x0, y0, z0, vx, vy, vz, phase, h, a, s,
+                                                                           *origin))))
+
+    def add_vortex_ring(self, m, OM, x0, y0, z0, thetax, thetay, thetaz, vx, vy, vz, phase, h, a, s, origin, d,
+                        opt_2D=0, ea=1.0, eb=1.0):
+        """NLSE_add_vortex_ring (NLSE_add_vortex_ring.m:10-534, opt_2D = 0) on
+        the resident 3D state."""
+        _check(self._lib.nlse_add_vortex_ring(self._h, int(m), *map(float, (OM, x0, y0, z0, thetax, thetay, thetaz, vx,
+                                                                           vy, vz, phase, h, a, s, *origin, d)),
+                                              int(opt_2D), float(ea), float(eb)))
+
     def profile_launches(self, steps):
         """Mean device ms of each kernel launch of an RK4 step (HIP events)."""
         out = (C.c_double * 4)()
@@ -551,6 +602,63 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def vortex_profile(m, OM, a, s, hr, n, ansatz, optimize=True, tol=1e-14):
+    """Radial vortex profile f(q*hr), q < n, of the discrete steady state
+    (NLS_STEADY_F.m:10-55); ansatz 0 dark sqrt, 1 dark tanh, 2 bright sech.
+    Host computation."""
+    f = np.empty(int(n))
+    _check(lib().nlse_vortex_profile(int(m), float(OM), float(a), float(s), float(hr), int(n), int(ansatz),
+                                     int(bool(optimize)), float(tol), _p(f, C.c_double)))
+    return f
+
+
+def _with_state(U, fn, dim):
+    """Run fn(solver) on a temporary resident FP64 state holding U; return the new U."""
+    U = np.asarray(U)
+    if U.ndim != dim:
+        raise ValueError(f"expected a {dim}D array, got shape {U.shape}")
+    S = Solver(U.shape, 1, 2, 1, -1.0, 1.0, 1.0, 1.0)
+    try:
+        S.set_state(U)
+        fn(S)
+        return S.state()
+    finally:
+        S.close()
+
+
+def _axis_origin(X, Y, Z=None):
+    if Z is None:
+        return (float(np.asarray(X).flat[0]), float(np.asarray(Y).flat[0]))
+    return (float(np.asarray(X).flat[0]), float(np.asarray(Y).flat[0]), float(np.asarray(Z).flat[0]))
+
+
+def NLS_add_vortex(charge, om, x0, y0, vx, vy, phase, U, h, a, s, amp, X, Y, opt):
+    """``U = NLS_add_vortex(charge,om,x0,y0,vx,vy,phase,U,h,a,s,amp,X,Y,opt)``
+    on the GPU (NLSEmagic2D_Full_Research_Scripts/NLS_add_vortex.m:9-158).
+    X, Y: meshgrid arrays in this package's layout (shape (N, M), x along
+    the first axis, y along the second); only their origin is read, the
+    grid is taken as uniform with spacing h."""
+    o = _axis_origin(X, Y)
+    return _with_state(U, lambda S: S.add_vortex_2d(charge, om, x0, y0, vx, vy, phase, h, a, s, amp, o, opt), 2)
+
+
+def NLSE_add_vortex_line(m, OM, x0, y0, z0, vx, vy, vz, phase, U, h, a, s, amp, X, Y, Z):
+    """``U = NLSE_add_vortex_line(m,OM,x0,y0,z0,vx,vy,vz,phase,U,h,a,s,amp,X,Y,Z)``
+    on the GPU (NLSEmagic3D/NLSE_add_vortex_line.m:10-125); arrays of shape
+    (L, N, M) with z, x, y along the axes."""
+    o = _axis_origin(X, Y, Z)
+    return _with_state(U, lambda S: S.add_vortex_line(m, OM, x0, y0, z0, vx, vy, vz, phase, h, a, s, o), 3)
+
+
+def NLSE_add_vortex_ring(m, OM, x0, y0, z0, thetax, thetay, thetaz, vx, vy, vz, phase, U, h, a, s, X, Y, Z, d,
+                         opt_2D, ea, eb):
+    """``U = NLSE_add_vortex_ring(m,OM,x0,y0,z0,thetax,thetay,thetaz,vx,vy,vz,phase,U,h,a,s,X,Y,Z,d,opt_2D,ea,eb)``
+    on the GPU (NLSEmagic3D/NLSE_add_vortex_ring.m:10-534) for opt_2D = 0."""
+    o = _axis_origin(X, Y, Z)
+    return _with_state(U, lambda S: S.add_vortex_ring(m, OM, x0, y0, z0, thetax, thetay, thetaz, vx, vy, vz, phase, h,
+                                                      a, s, o, d, opt_2D, ea, eb), 3)
 
 
 def copy_probe(nbytes, reps=10):

@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnlsemagic_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["nlse1d.hip", "nlse2d.hip", "nlse2d_batch.hip", "nlse3d.hip", "nlse_capi.hip", "nlse_batch.hip", "nlse_diag.hip",
-           "nlse_vort.hip"]
+           "nlse_vort.hip", "nlse_ic.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NLSE_OFFLOAD_ARCH", "gfx950")

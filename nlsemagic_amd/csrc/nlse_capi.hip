@@ -333,6 +333,20 @@ int download_comp(nlse_solver* h, double* Ur, double* Ui)
     return NLSE_OK;
 }
 
+IcGrid ic_grid(const nlse_solver* h, double xmin, double ymin, double zmin, double hg)
+{
+    IcGrid q;
+    q.dim = h->dim;
+    q.N = h->N;
+    q.M = h->M;
+    q.L = h->L;
+    q.xmin = xmin;
+    q.ymin = ymin;
+    q.zmin = zmin;
+    q.h = hg;
+    return q;
+}
+
 // ---- drop-in entry points: one cached solver per shape/precision
 
 std::mutex g_cache_mu;
@@ -601,6 +615,70 @@ int nlse_vorticity(nlse_solver* h, double lh2, double veps, double* wx, double* 
     if (buf) (void)hipFree(buf);
     if (rc == NLSE_OK && e != hipSuccess) rc = fail(NLSE_EHIP, std::string("vorticity: ") + hipGetErrorString(e));
     return rc;
+}
+
+int nlse_set_exact(nlse_solver* h, int kind, const double* par, double t, double xmin, double ymin, double zmin,
+                   double hgrid)
+{
+    if (!h || !par) return fail(NLSE_EINVAL, "null argument");
+    const IcGrid q = ic_grid(h, xmin, ymin, zmin, hgrid);
+    return is_double(h) ? ic_set_exact<double>((double2*)h->U, nullptr, (double*)h->V, q, kind, par, t, h->stream)
+                        : ic_set_exact<float>((float2*)h->U, (float2*)h->Ulo, (float*)h->V, q, kind, par, t, h->stream);
+}
+
+int nlse_exact_error(nlse_solver* h, int kind, const double* par, double t, double xmin, double ymin, double zmin,
+                     double hgrid, double* out6)
+{
+    if (!h || !par || !out6) return fail(NLSE_EINVAL, "null argument");
+    const IcGrid q = ic_grid(h, xmin, ymin, zmin, hgrid);
+    return is_double(h) ? ic_exact_error<double>((const double2*)h->U, nullptr, q, kind, par, t, out6, h->stream)
+                        : ic_exact_error<float>((const float2*)h->U, (const float2*)h->Ulo, q, kind, par, t, out6,
+                                                h->stream);
+}
+
+int nlse_add_vortex_2d(nlse_solver* h, int charge, double om, double x0, double y0, double vx, double vy,
+                       double phase, double hgrid, double a, double s, double amp, double xmin, double ymin, int opt)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    const IcGrid q = ic_grid(h, xmin, ymin, 0, hgrid);
+    return is_double(h) ? ic_vortex_2d<double>((double2*)h->U, nullptr, q, charge, om, x0, y0, vx, vy, phase, a, s,
+                                               amp, opt, h->stream)
+                        : ic_vortex_2d<float>((float2*)h->U, (float2*)h->Ulo, q, charge, om, x0, y0, vx, vy, phase,
+                                              a, s, amp, opt, h->stream);
+}
+
+int nlse_add_vortex_line(nlse_solver* h, int m, double OM, double x0, double y0, double z0, double vx, double vy,
+                         double vz, double phase, double hgrid, double a, double s, double xmin, double ymin,
+                         double zmin)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    const IcGrid q = ic_grid(h, xmin, ymin, zmin, hgrid);
+    return is_double(h) ? ic_vortex_line<double>((double2*)h->U, nullptr, q, m, OM, x0, y0, z0, vx, vy, vz, phase,
+                                                 a, s, h->stream)
+                        : ic_vortex_line<float>((float2*)h->U, (float2*)h->Ulo, q, m, OM, x0, y0, z0, vx, vy, vz,
+                                                phase, a, s, h->stream);
+}
+
+int nlse_add_vortex_ring(nlse_solver* h, int m, double OM, double x0, double y0, double z0, double thetax,
+                         double thetay, double thetaz, double vx, double vy, double vz, double phase, double hgrid,
+                         double a, double s, double xmin, double ymin, double zmin, double d, int opt_2D, double ea,
+                         double eb)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    const IcGrid q = ic_grid(h, xmin, ymin, zmin, hgrid);
+    return is_double(h) ? ic_vortex_ring<double>((double2*)h->U, nullptr, q, m, OM, x0, y0, z0, thetax, thetay,
+                                                 thetaz, vx, vy, vz, phase, a, s, d, opt_2D, ea, eb, h->stream)
+                        : ic_vortex_ring<float>((float2*)h->U, (float2*)h->Ulo, q, m, OM, x0, y0, z0, thetax, thetay,
+                                                thetaz, vx, vy, vz, phase, a, s, d, opt_2D, ea, eb, h->stream);
+}
+
+int nlse_vortex_profile(int m, double OM, double a, double s, double hr, int n, int ansatz, int optimize, double tol,
+                        double* f)
+{
+    if (!f) return fail(NLSE_EINVAL, "null argument");
+    if (ansatz < 0 || ansatz > 2) return fail(NLSE_EINVAL, "ansatz must be 0 (dark sqrt), 1 (dark tanh) or 2 (bright sech)");
+    if (s == 0) return fail(NLSE_EINVAL, "s must be nonzero");
+    return vortex_profile(m, OM, a, s, hr, n, ansatz, optimize, tol, f);
 }
 
 int nlse_query(nlse_solver* h)

@@ -136,6 +136,31 @@ hipError_t launch_vorticity3d(const c2<T>* u, const c2<T>* ulo, int L, int N, in
                               double* vx, double* vy, double* vz, double* wx, double* wy, double* wz, double* W,
                               hipStream_t st);
 
+// closed-form fields, exact-solution error norms and vortex initial
+// conditions of a state (nlse_ic.hip); grid: dims and origin (xmin, ymin,
+// zmin) with spacing h.  All synchronous, returning NLSE_* codes.
+struct IcGrid {
+    int dim, N, M, L;
+    double xmin, ymin, zmin, h;
+};
+int vortex_profile(int m, double OM, double a, double s, double hr, int n, int ansatz, int optimize, double tol,
+                   double* out);
+template <typename T>
+int ic_set_exact(c2<T>* u, c2<T>* ulo, T* V, const IcGrid& q, int kind, const double* par, double t, hipStream_t st);
+template <typename T>
+int ic_exact_error(const c2<T>* u, const c2<T>* ulo, const IcGrid& q, int kind, const double* par, double t,
+                   double* out6, hipStream_t st);
+template <typename T>
+int ic_vortex_2d(c2<T>* u, c2<T>* ulo, const IcGrid& q, int m, double om, double x0, double y0, double vx, double vy,
+                 double phase, double a, double s, double amp, int opt, hipStream_t st);
+template <typename T>
+int ic_vortex_line(c2<T>* u, c2<T>* ulo, const IcGrid& q, int m, double OM, double x0, double y0, double z0,
+                   double vx, double vy, double vz, double phase, double a, double s, hipStream_t st);
+template <typename T>
+int ic_vortex_ring(c2<T>* u, c2<T>* ulo, const IcGrid& q, int m, double OM, double x0, double y0, double z0,
+                   double thetax, double thetay, double thetaz, double vx, double vy, double vz, double phase,
+                   double a, double s, double d, int opt_2D, double ea, double eb, hipStream_t st);
+
 // split double planes into the hi + lo single-precision pair of a compensated state
 void split_hilo(const double* x, float* hi, float* lo, long n);
 
