@@ -350,7 +350,7 @@ def run_serving(args):
     for r in warm:
         r.steps = 20
         r.shape = tuple(min(n, 64) for n in r.shape)
-        r.make_state = S._dark_state(r.shape, 0.1)
+        r.init = S.default_ic(r.shape, 0.1)
     S.Server(max_active=8, chunk_steps=10).run(warm)
     reqs = S.request_mix(args.requests, qps=args.qps, seed=2)
     srv = S.Server(budget_bytes=200e9, max_active=16, chunk_steps=200)
@@ -369,7 +369,8 @@ def run_serving(args):
         "ms_per_step": stats["makespan_s"] * 1e3 / len(reqs), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64 + f32c (per request)", "data": "synthetic",
         "config": {"workload": f"serving: {len(reqs)} Poisson requests at {args.qps} QPS (seed 2), mixed 1D/2D/3D "
-                               "CD/2SHOC FP64/FP32c, 100-20000 steps, 200 GB budget",
+                               "CD/2SHOC FP64/FP32c, 100-20000 steps, 200 GB budget, initial states built on the "
+                               "device from parameters",
                    "server": stats, "one_at_a_time_fp64": plain,
                    "latency_p50_s": stats["latency_p50_s"], "latency_p99_s": stats["latency_p99_s"],
                    "latency_p50_by_precision": {("fp64" if p == 2 else "fp32c"): float(np.percentile(v, 50))

@@ -156,6 +156,9 @@ int nlse_set_exact(nlse_solver *h, int kind, const double *par, double t, double
                    double zmin, double hgrid);
 int nlse_exact_error(nlse_solver *h, int kind, const double *par, double t, double xmin, double ymin,
                      double zmin, double hgrid, double *out6);
+/* U = re + i im at every point (V unchanged): the uniform background the
+ * vortex generators are combined into (NLSE3D.m:405-407) */
+int nlse_fill_state(nlse_solver *h, double re, double im);
 /* Vortex initial conditions combined into the state as the reference's
  * generators do (s < 0: the normalised fields are multiplied and rescaled to
  * the new vortex's amplitude; s > 0: the new field is added).  The radial

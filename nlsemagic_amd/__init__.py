@@ -135,6 +135,8 @@ def lib():
     L.nlse_vorticity_device.argtypes = [C.c_void_p, C.c_double, C.c_double] + [C.c_void_p] * 4
     L.nlse_set_exact.restype = C.c_int
     L.nlse_set_exact.argtypes = [C.c_void_p, C.c_int, _D] + [C.c_double] * 5
+    L.nlse_fill_state.restype = C.c_int
+    L.nlse_fill_state.argtypes = [C.c_void_p, C.c_double, C.c_double]
     L.nlse_exact_error.restype = C.c_int
     L.nlse_exact_error.argtypes = [C.c_void_p, C.c_int, _D] + [C.c_double] * 5 + [_D]
     L.nlse_add_vortex_2d.restype = C.c_int
@@ -372,6 +374,11 @@ class Solver:
         par = np.array([a, s, OM, c, x0], dtype=np.float64)
         o = tuple(origin) + (0.0,) * (3 - len(tuple(origin)))
         _check(self._lib.nlse_set_exact(self._h, int(kind), _p(par, C.c_double), float(t), *map(float, o), float(h)))
+
+    def fill(self, value):
+        """U = value at every point (V unchanged)."""
+        value = complex(value)
+        _check(self._lib.nlse_fill_state(self._h, value.real, value.imag))
 
     def exact_error(self, kind, h, origin, t, a=1.0, s=0.0, OM=0.0, c=0.0, x0=0.0):
         """(rms_re, rms_im, rms_mod2, max_re, max_im, max_mod2) of the state

@@ -626,6 +626,13 @@ int nlse_set_exact(nlse_solver* h, int kind, const double* par, double t, double
                         : ic_set_exact<float>((float2*)h->U, (float2*)h->Ulo, (float*)h->V, q, kind, par, t, h->stream);
 }
 
+int nlse_fill_state(nlse_solver* h, double re, double im)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    return is_double(h) ? ic_fill<double>((double2*)h->U, nullptr, h->n, re, im, h->stream)
+                        : ic_fill<float>((float2*)h->U, (float2*)h->Ulo, h->n, re, im, h->stream);
+}
+
 int nlse_exact_error(nlse_solver* h, int kind, const double* par, double t, double xmin, double ymin, double zmin,
                      double hgrid, double* out6)
 {

@@ -148,6 +148,8 @@ int vortex_profile(int m, double OM, double a, double s, double hr, int n, int a
 template <typename T>
 int ic_set_exact(c2<T>* u, c2<T>* ulo, T* V, const IcGrid& q, int kind, const double* par, double t, hipStream_t st);
 template <typename T>
+int ic_fill(c2<T>* u, c2<T>* ulo, long n, double re, double im, hipStream_t st);
+template <typename T>
 int ic_exact_error(const c2<T>* u, const c2<T>* ulo, const IcGrid& q, int kind, const double* par, double t,
                    double* out6, hipStream_t st);
 template <typename T>

@@ -131,6 +131,14 @@ __global__ __launch_bounds__(256) void set_exact_kernel(c2<T>* __restrict__ u, c
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(c2<T>* __restrict__ u, c2<T>* __restrict__ ulo, long n, double re,
+                                                   double im)
+{
+    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < n; p += (long)gridDim.x * 256)
+        store_state<T>(u, ulo, p, make_double2(re, im));
+}
+
 // deterministic block reduction of NS sums and NM maxima
 template <int NS, int NM>
 __device__ void block_reduce(double (&s)[NS], double (&m)[NM], double* out)
@@ -549,6 +557,15 @@ int ic_set_exact(c2<T>* u, c2<T>* ulo, T* V, const IcGrid& q, int kind, const do
 }
 
 template <typename T>
+int ic_fill(c2<T>* u, c2<T>* ulo, long n, double re, double im, hipStream_t st)
+{
+    hipLaunchKernelGGL((fill_kernel<T>), dim3(ic_blocks(n)), dim3(256), 0, st, u, ulo, n, re, im);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    return NLSE_OK;
+}
+
+template <typename T>
 int ic_exact_error(const c2<T>* u, const c2<T>* ulo, const IcGrid& q, int kind, const double* par, double t,
                    double* out6, hipStream_t st)
 {
@@ -801,6 +818,7 @@ int ic_vortex_ring(c2<T>* u, c2<T>* ulo, const IcGrid& q, int m, double OM, doub
 
 #define NLSE_IC_INST(T)                                                                                              \
     template int ic_set_exact<T>(c2<T>*, c2<T>*, T*, const IcGrid&, int, const double*, double, hipStream_t);        \
+    template int ic_fill<T>(c2<T>*, c2<T>*, long, double, double, hipStream_t);                                      \
     template int ic_exact_error<T>(const c2<T>*, const c2<T>*, const IcGrid&, int, const double*, double, double*,   \
                                    hipStream_t);                                                                     \
     template int ic_vortex_2d<T>(c2<T>*, c2<T>*, const IcGrid&, int, double, double, double, double, double, double, \

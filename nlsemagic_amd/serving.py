@@ -48,7 +48,8 @@ class Request:
     h2: float
     k: float
     steps: int
-    make_state: Callable[[], tuple]  # -> (U0, V)
+    make_state: Optional[Callable[[], tuple]] = None  # -> (U0, V) host arrays, uploaded
+    init: Optional[Callable[[Solver], None]] = None   # builds the state on the device instead
     admitted: Optional[float] = None
     finished: Optional[float] = None
     result: Optional[np.ndarray] = None
@@ -66,6 +67,26 @@ class Request:
     def latency(self) -> float:
         return self.finished - self.arrival
 
+    def initialize(self, sol: Solver):
+        """Put the initial state into a fresh solver: on the device from the
+        request's parameters (init) or from host arrays (make_state)."""
+        if self.init is not None:
+            self.init(sol)
+        else:
+            U0, V = self.make_state()
+            sol.set_state(U0, V)
+
+    def initial_state(self):
+        """(U0, V) on the host, for a path that takes arrays."""
+        if self.init is None:
+            return self.make_state()
+        sol = Solver(self.shape, self.method, 2, self.BC, self.s, self.a, self.h2, self.k)
+        try:
+            self.init(sol)
+            return sol.state(), np.zeros(self.shape)
+        finally:
+            sol.close()
+
 
 class Server:
     def __init__(self, budget_bytes: float = 200e9, max_active: int = 16, chunk_steps: int = 200,
@@ -80,8 +101,7 @@ class Server:
     def _admit(self, r: Request, now: float):
         sol = Solver(r.shape, r.method, r.precision, r.BC, r.s, r.a, r.h2, r.k)
         sol.set_graph(min(self.chunk, r.steps))
-        U0, V = r.make_state()
-        sol.set_state(U0, V)
+        r.initialize(sol)
         r.admitted = now
         return sol
 
@@ -146,7 +166,7 @@ def run_one_at_a_time(requests: List[Request], precision: Optional[int] = 2) -> 
         while time.perf_counter() - t0 < r.arrival:
             time.sleep(min(0.01, r.arrival - (time.perf_counter() - t0)))
         r.admitted = time.perf_counter() - t0
-        U0, V = r.make_state()
+        U0, V = r.initial_state()
         prec = r.precision if precision is None else precision
         if prec == 3:
             prec = 1
@@ -205,9 +225,9 @@ def request_mix(n: int, qps: float, seed: int = 2) -> List[Request]:
     log-uniform in [100, 20000].  Per request, from one PCG64 stream seeded
     with `seed`, in this order: inter-arrival gap, dim, method, precision,
     size, steps.  s = -1, a = 1, V = 0, Dirichlet boundaries, k = 0.8 klin;
-    h = 0.1 (1D, 2D) or 0.2 (3D); the initial state is a dark soliton (1D),
-    a dark vortex (2D) or a dark vortex ring of radius a quarter of the box
-    (3D)."""
+    h = 0.1 (1D, 2D) or 0.2 (3D); the initial state arrives as parameters
+    and is built on the device (default_ic): a dark soliton (1D), a dark
+    vortex (2D) or a dark vortex ring of radius a quarter of the box (3D)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -228,22 +248,34 @@ def request_mix(n: int, qps: float, seed: int = 2) -> List[Request]:
         steps = int(round(lu(100, 20000)))
         h = 0.1 if dim < 3 else 0.2
         out.append(Request(i, t, shape, method, precision, 1, -1.0, 1.0, h * h, auto_k(dim, method, h, 1.0),
-                           steps, _dark_state(shape, h)))
+                           steps, init=default_ic(shape, h)))
     return out
 
 
-def _dark_state(shape, h):
-    def make():
-        grids = np.meshgrid(*[h * (np.arange(n) - (n - 1) / 2) for n in shape], indexing="ij")
-        if len(shape) == 1:
-            U = np.tanh(grids[0] / np.sqrt(2)) + 0j
-        elif len(shape) == 2:
-            X, Y = grids
-            U = np.tanh(np.hypot(X, Y)) * np.exp(1j * np.arctan2(Y, X))
-        else:
-            Z, Y, X = grids
-            R = 0.25 * h * shape[-1]
-            rho = np.hypot(X, Y)
-            U = np.tanh(np.hypot(rho - R, Z) / np.sqrt(2)) * np.exp(1j * np.arctan2(Z, rho - R))
-        return U, np.zeros(shape)
-    return make
+def default_ic(shape, h):
+    """The serving workload's initial state, built on the device by the
+    reference's generators on the centred grid of spacing h (s = -1, a = 1,
+    background amplitude 1):
+      1D  the dark soliton tanh(x/sqrt 2) (NLSEmagic1D_FRS.m:207, c = 0);
+      2D  a charge-1 dark vortex at the centre (NLS_add_vortex.m, opt = 0:
+          tanh(r/sqrt 2) e^{i theta});
+      3D  a charge-1 dark vortex ring of radius a quarter of the box in the
+          x-y plane (NLSE_add_vortex_ring.m, opt_2D = 0) on the uniform
+          background (NLSE3D.m:405-407)."""
+    origin = tuple(-(n - 1) * h / 2 for n in shape)
+    if len(shape) == 1:
+        return lambda sol: sol.set_exact(2, h, origin, 0.0, a=1.0, s=-1.0, OM=-1.0, c=0.0, x0=0.0)
+    if len(shape) == 2:
+        o2 = (origin[0], origin[1])
+
+        def init2(sol):
+            sol.fill(1.0)
+            sol.add_vortex_2d(1, -1.0, 0.0, 0.0, 0.0, 0.0, 0.0, h, 1.0, -1.0, 1.0, o2, 0)
+        return init2
+    L, N, M = shape
+    o3 = (origin[1], origin[2], origin[0])  # (xmin, ymin, zmin): x along N, y along M, z along L
+
+    def init3(sol):
+        sol.fill(1.0)
+        sol.add_vortex_ring(1, -1.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0, 0, 0.0, h, 1.0, -1.0, o3, 0.25 * h * M)
+    return init3

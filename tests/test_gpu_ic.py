@@ -214,3 +214,41 @@ def test_generators_reject_bad_arguments():
         S2.add_vortex_line(1, -1.0, 0, 0, 0, 0, 0, 0, 0, 0.3, 1.0, -1.0, (-1, -1, -1))
     S2.close()
     S3.close()
+
+
+@pytest.mark.parametrize("precision", [2, 1, 3])
+def test_fill_state(precision):
+    S = _solver((5, 6, 7), precision)
+    S.fill(0.3 - 0.7j)
+    got = S.state()
+
+    def split(x):  # the value a state of this precision holds (hi + lo for the compensated one)
+        hi = np.float32(x)
+        return float(hi) + (float(np.float32(x - float(hi))) if precision == 3 else 0.0)
+    exp = complex(split(0.3), split(-0.7)) if precision != 2 else 0.3 - 0.7j
+    assert np.all(got == exp)
+    S.close()
+
+
+def test_serving_default_ics_are_the_generators():
+    """The serving workload's device-built states: 1D dark soliton, 2D
+    vortex, 3D ring (oracle restatements of the same generator calls)."""
+    from nlsemagic_amd import serving as SV
+    h = 0.2
+    for shape in [(101,), (30, 26), (24, 24, 24)]:
+        S = _solver(shape, 2)
+        SV.default_ic(shape, h)(S)
+        got = S.state()
+        S.close()
+        origin = tuple(-(n - 1) * h / 2 for n in shape)
+        one = np.ones(shape, complex)
+        if len(shape) == 1:
+            exp = I.exact(2, (1.0, -1.0, -1.0, 0.0, 0.0), shape, origin, h, 0.0)
+        elif len(shape) == 2:
+            exp = I.add_vortex_2d(1, -1.0, 0.0, 0.0, 0.0, 0.0, 0.0, one, h, 1.0, -1.0, 1.0, origin, 0)
+        else:
+            o3 = (origin[1], origin[2], origin[0])
+            exp = I.add_vortex_ring(1, -1.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0, 0, 0.0, one, h, 1.0, -1.0, o3,
+                                    0.25 * h * shape[2])
+        print("CASE serving ic", shape, flush=True)
+        assert np.max(np.abs(got - exp)) < 1e-11

@@ -62,7 +62,7 @@ def _small_mix(seed=5, n=10):
         d = len(r.shape)
         r.shape = {1: (4100,), 2: (70, 90), 3: (20, 22, 24)}[d]
         r.steps = 30 + 17 * r.rid
-        r.make_state = S._dark_state(r.shape, 0.1 if d < 3 else 0.2)
+        r.init = S.default_ic(r.shape, 0.1 if d < 3 else 0.2)
     return reqs
 
 
@@ -74,8 +74,7 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
     assert stats["peak_active"] <= 4
     for r in reqs:
         sol = nm.Solver(r.shape, r.method, r.precision, r.BC, r.s, r.a, r.h2, r.k)
-        U0, V = r.make_state()
-        sol.set_state(U0, V)
+        r.initialize(sol)
         sol.step(r.steps)
         alone = sol.state()
         sol.close()
