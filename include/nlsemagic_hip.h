@@ -130,8 +130,7 @@ int nlse_diagnostics(nlse_solver *h, double *out3);
  * Needs a 3D solver with >= 3 points along every axis; synchronous. */
 int nlse_vorticity(nlse_solver *h, double lh2, double veps, double *wx, double *wy, double *wz, double *W);
 /* the same into device arrays (n doubles each, any may be NULL), enqueued
- * on the solver's stream without allocating (the velocity is staged in the
- * solver's RK4 work buffers) */
+ * on the solver's stream without allocating (one kernel, no scratch) */
 int nlse_vorticity_device(nlse_solver *h, double lh2, double veps, double *wx, double *wy, double *wz,
                           double *W);
 /* ---- closed-form fields and initial conditions of the resident state ----

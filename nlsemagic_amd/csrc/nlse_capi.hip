@@ -583,14 +583,11 @@ int nlse_vorticity_device(nlse_solver* h, double lh2, double veps, double* wx, d
     if (!h) return fail(NLSE_EINVAL, "null solver");
     if (h->dim != 3) return fail(NLSE_EINVAL, "vorticity is defined for 3D solvers");
     if (h->N < 3 || h->M < 3 || h->L < 3) return fail(NLSE_EINVAL, "vorticity needs >= 3 points along every axis");
-    // the velocity goes into the RK4 work buffers Y0, Y1, ktot (>= 8 bytes per
-    // point each), free between steps: a 3D solver's state stays in U
-    double *vx = (double*)h->Y0, *vy = (double*)h->Y1, *vz = (double*)h->acc;
     hipError_t e = is_double(h)
-        ? launch_vorticity3d<double>((const double2*)h->U, nullptr, h->L, h->N, h->M, lh2, veps, vx, vy, vz, wx, wy,
-                                     wz, W, h->stream)
-        : launch_vorticity3d<float>((const float2*)h->U, (const float2*)h->Ulo, h->L, h->N, h->M, lh2, veps, vx, vy,
-                                    vz, wx, wy, wz, W, h->stream);
+        ? launch_vorticity3d<double>((const double2*)h->U, nullptr, h->L, h->N, h->M, lh2, veps, wx, wy, wz, W,
+                                     h->stream)
+        : launch_vorticity3d<float>((const float2*)h->U, (const float2*)h->Ulo, h->L, h->N, h->M, lh2, veps, wx, wy,
+                                    wz, W, h->stream);
     if (e != hipSuccess) return fail(NLSE_EHIP, std::string("vorticity: ") + hipGetErrorString(e));
     return NLSE_OK;
 }
@@ -898,22 +895,19 @@ int NLSE3D_VORTICITY_HIP(const double* Ur, const double* Ui, double lh2, double 
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NLSE_EHIP, "no HIP device available");
     const long n = (long)N * M * L;
     const size_t b = sizeof(double) * (size_t)n;
-    // U (2n), velocity (3n, also the upload staging area), wx, wy, wz, W (4n)
+    // U (2n), wx, wy, wz, W (4n); the outputs double as the upload staging area
     double* buf = nullptr;
-    if (hipMalloc((void**)&buf, b * 9) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    if (hipMalloc((void**)&buf, b * 6) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(buf);
         return fail(NLSE_EHIP, "stream creation failed");
     }
     double2* u = (double2*)buf;
-    double* v = buf + 2 * n;
-    double* w = buf + 5 * n;
-    int rc = upload_planes<double>(Ur, Ui, u, v, n, st);
+    double* w = buf + 2 * n;
+    int rc = upload_planes<double>(Ur, Ui, u, w, n, st);
     hipError_t e = hipSuccess;
-    if (rc == NLSE_OK)
-        e = launch_vorticity3d<double>(u, nullptr, L, N, M, lh2, veps, v, v + n, v + 2 * n, w, w + n, w + 2 * n,
-                                       w + 3 * n, st);
+    if (rc == NLSE_OK) e = launch_vorticity3d<double>(u, nullptr, L, N, M, lh2, veps, w, w + n, w + 2 * n, w + 3 * n, st);
     double* host[4] = {wx, wy, wz, W};
     for (int o = 0; o < 4 && rc == NLSE_OK && e == hipSuccess; ++o)
         e = hipMemcpyAsync(host[o], w + (size_t)o * n, b, hipMemcpyDeviceToHost, st);

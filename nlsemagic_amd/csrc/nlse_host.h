@@ -127,14 +127,12 @@ int download_planes(const c2<T>* src, T* Ur, T* Ui, T* stage, long n, hipStream_
 // diagnostics of n points (u, plus ulo for a compensated state) into out[0..2]
 // on the device: sum |U|^2, max Re U, non-finite count; part: 3*1024 doubles
 hipError_t launch_diag(const void* u, const void* ulo, bool dbl, long n, double* part, double* out, hipStream_t st);
-// 3D vorticity of a state (L x N x M, k fastest) in double precision:
-// velocity into vx, vy, vz (n doubles each), then wx, wy, wz, W (device
-// pointers, any may be null); ulo: low part of a compensated single-precision
-// state (or null).
+// 3D vorticity of a state (L x N x M, k fastest) in double precision into
+// wx, wy, wz, W (device pointers, any may be null); ulo: low part of a
+// compensated single-precision state (or null).  No scratch memory.
 template <typename T>
 hipError_t launch_vorticity3d(const c2<T>* u, const c2<T>* ulo, int L, int N, int M, double lh2, double veps,
-                              double* vx, double* vy, double* vz, double* wx, double* wy, double* wz, double* W,
-                              hipStream_t st);
+                              double* wx, double* wy, double* wz, double* W, hipStream_t st);
 
 // closed-form fields, exact-solution error norms and vortex initial
 // conditions of a state (nlse_ic.hip); grid: dims and origin (xmin, ymin,

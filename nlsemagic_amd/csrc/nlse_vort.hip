@@ -8,15 +8,22 @@
 // with central differences lh2*(A[+1] - A[-1]) at interior points and
 // one-sided (2 lh2)*(A[+1] - A[0]) / (2 lh2)*(A[0] - A[-1]) normal to a face
 // (the y- and z-faces also take the x-derivative one-sided forward, as the
-// reference does).  Two passes: velocity (one point per thread, the U
-// neighbours through L1/L2), then the curl of V; workgroups of 64 x 4
-// points of one plane.  All arithmetic is double,
-// in the reference's operand order (FP contraction off, correctly rounded
-// division and square root), so the result is bit-identical to the
-// reference wherever the reference is defined: it never sets the gradient on
-// edges and corners (two or more boundary indices); here it is 0 there, and
-// the curl on edges and corners is 0 as in the reference's zero-initialised
-// outputs.  Single-precision states are widened (hi + lo) to double first.
+// reference does).  All arithmetic is double, in the reference's operand
+// order (FP contraction off, correctly rounded division and square root), so
+// the result is bit-identical to the reference wherever the reference is
+// defined: it never sets the gradient on edges and corners (two or more
+// boundary indices); here it is 0 there, and the curl on edges and corners
+// is 0 as in the reference's zero-initialised outputs.  Single-precision
+// states are widened (hi + lo) to double first.
+//
+// One kernel, no scratch: a workgroup owns TJ rows (j) x 60 columns (k) and
+// marches along i over a chunk of planes.  Lane l is column k0 - 2 + l;
+// extended row e is row j0 - 2 + e and belongs to wave e mod 4.  Per plane
+// p: U of plane p enters an LDS ring of three planes (widened to double),
+// the velocity of plane p-1 is formed from the ring into an LDS ring of
+// three velocity planes, and the curl of plane p-2 is formed from that ring
+// and written.  HBM traffic per point: U once (plus halo re-reads through
+// L2) and the four outputs.
 #include "nlse_host.h"
 
 namespace nlse {
@@ -28,28 +35,8 @@ struct Dims3 {
     long si, sj;
 };
 
-// launch shape: 64 columns (k) x 4 rows (j) per workgroup, one plane (i)
-// per grid z; no index arithmetic beyond multiplications
-constexpr int VT_K = 64, VT_J = 4;
-
-__device__ __forceinline__ bool point3(const Dims3& g, int& i, int& j, int& k, long& p)
-{
-    k = blockIdx.x * VT_K + threadIdx.x;
-    j = blockIdx.y * VT_J + threadIdx.y;
-    i = blockIdx.z;
-    p = (long)i * g.si + (long)j * g.sj + k;
-    return k < g.M && j < g.N;
-}
-
-// derivative of A along a stride: central, one-sided forward or backward
+// difference kinds: central, one-sided forward or backward
 enum { D_CEN = 0, D_FWD = 1, D_BWD = 2 };
-template <typename Get>
-__device__ __forceinline__ double deriv(Get A, long p, long s, int kind, double lh2)
-{
-    if (kind == D_FWD) return (lh2 * 2) * (A(p + s) - A(p));
-    if (kind == D_BWD) return (lh2 * 2) * (A(p) - A(p - s));
-    return lh2 * (A(p + s) - A(p - s));
-}
 
 // difference kinds (x, y, z) at (i, j, k); false on edges and corners
 __device__ __forceinline__ bool kinds3(int i, int j, int k, const Dims3& g, int& kx, int& ky, int& kz)
@@ -70,100 +57,197 @@ __device__ __forceinline__ bool kinds3(int i, int j, int k, const Dims3& g, int&
 }
 
 template <typename T>
-__device__ __forceinline__ void load_u(const c2<T>* u, const c2<T>* ulo, long p, double& ur, double& ui)
+__device__ __forceinline__ double2 load_u(const c2<T>* u, const c2<T>* ulo, long p)
 {
     const c2<T> h = u[p];
-    ur = (double)h.x;
-    ui = (double)h.y;
+    double2 r = make_double2((double)h.x, (double)h.y);
     if (ulo) {
         const c2<T> l = ulo[p];
-        ur = ur + (double)l.x;
-        ui = ui + (double)l.y;
+        r.x = r.x + (double)l.x;
+        r.y = r.y + (double)l.y;
     }
+    return r;
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void vel3d_kernel(const c2<T>* __restrict__ u, const c2<T>* __restrict__ ulo,
-                                                    double lh2, double veps, Dims3 g, double* __restrict__ Vx,
-                                                    double* __restrict__ Vy, double* __restrict__ Vz)
-{
-    int i, j, k, kx, ky, kz;
-    long p;
-    if (point3(g, i, j, k, p)) {
-        double gxr = 0, gxi = 0, gyr = 0, gyi = 0, gzr = 0, gzi = 0;
-        if (kinds3(i, j, k, g, kx, ky, kz)) {
-            auto Ur = [&](long q) { double a, b; load_u<T>(u, ulo, q, a, b); return a; };
-            auto Ui = [&](long q) { double a, b; load_u<T>(u, ulo, q, a, b); return b; };
-            gxr = deriv(Ur, p, g.si, kx, lh2);
-            gxi = deriv(Ui, p, g.si, kx, lh2);
-            gyr = deriv(Ur, p, g.sj, ky, lh2);
-            gyi = deriv(Ui, p, g.sj, ky, lh2);
-            gzr = deriv(Ur, p, 1, kz, lh2);
-            gzi = deriv(Ui, p, 1, kz, lh2);
-        }
-        double ur, ui;
-        load_u<T>(u, ulo, p, ur, ui);
-        const double l_U2 = 1 / (2 * ur * ur + 2 * ui * ui + veps);
-        Vx[p] = (ur * gxi - ui * gxr) * l_U2;
-        Vy[p] = (ur * gyi - ui * gyr) * l_U2;
-        Vz[p] = (ur * gzi - ui * gzr) * l_U2;
-    }
-}
+struct VortMarch {
+    int IC;                 // planes per chunk
+    int tj, tk, ti;         // tiles along j, k and chunks along i
+    int per_xcd;            // workgroups per XCD
+};
 
-__global__ __launch_bounds__(256) void curl3d_kernel(const double* __restrict__ Vx, const double* __restrict__ Vy,
-                                                     const double* __restrict__ Vz, double lh2, Dims3 g,
-                                                     double* __restrict__ wx, double* __restrict__ wy,
-                                                     double* __restrict__ wz, double* __restrict__ W)
+template <int TJ>
+struct VortLds {
+    static constexpr int TE = TJ + 4;   // extended rows of U
+    double2 u[3][TE][64];               // U of planes p, p-1, p-2 (ring slot = step mod 3)
+    double v[3][3][TE - 2][64];         // velocity (x, y, z) of three planes, rows 1 .. TE-2
+};
+
+template <typename T, int TJ>
+__global__ __launch_bounds__(256) void vort_march_kernel(const c2<T>* __restrict__ u, const c2<T>* __restrict__ ulo,
+                                                         double lh2, double veps, Dims3 g, VortMarch vm,
+                                                         double* __restrict__ wx, double* __restrict__ wy,
+                                                         double* __restrict__ wz, double* __restrict__ W)
 {
-    int i, j, k, kx, ky, kz;
-    long p;
-    if (point3(g, i, j, k, p)) {
-        double ox = 0, oy = 0, oz = 0, on = 0;
-        if (kinds3(i, j, k, g, kx, ky, kz)) {
-            auto VX = [&](long q) { return Vx[q]; };
-            auto VY = [&](long q) { return Vy[q]; };
-            auto VZ = [&](long q) { return Vz[q]; };
-            const double vzx = deriv(VZ, p, g.si, kx, lh2), vyx = deriv(VY, p, g.si, kx, lh2);
-            const double vzy = deriv(VZ, p, g.sj, ky, lh2), vxy = deriv(VX, p, g.sj, ky, lh2);
-            const double vyz = deriv(VY, p, 1, kz, lh2), vxz = deriv(VX, p, 1, kz, lh2);
-            ox = vzy - vyz;
-            oy = vxz - vzx;
-            oz = vyx - vxy;
-            on = sqrt(ox * ox + oy * oy + oz * oz);
+    constexpr int TE = TJ + 4, RW = (TE + 3) / 4;
+    __shared__ VortLds<TJ> sm;
+    const int t = (blockIdx.x & 7) * vm.per_xcd + (blockIdx.x >> 3);
+    if (t >= vm.tj * vm.tk * vm.ti) return;
+    const int kt = t % vm.tk, rest = t / vm.tk;
+    const int jt = rest % vm.tj, ci = rest / vm.tj;
+    const int i0 = ci * vm.IC, j0 = jt * TJ, k0 = kt * 60;
+    const int iend = min(i0 + vm.IC, g.L);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int k = k0 - 2 + lane;
+    const int kc = k < 0 ? 0 : (k >= g.M ? g.M - 1 : k);
+    const bool kin = k >= 0 && k < g.M;
+
+    double2 un[RW];
+    auto load_plane = [&](int pl) {
+        const long pb = (long)(pl < 0 ? 0 : (pl >= g.L ? g.L - 1 : pl)) * g.si;
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int e = 4 * r + w;
+            const int j = j0 - 2 + e;
+            const int jc = j < 0 ? 0 : (j >= g.N ? g.N - 1 : j);
+            un[r] = load_u<T>(u, ulo, pb + (long)jc * g.sj + kc);
         }
-        if (wx) wx[p] = ox;
-        if (wy) wy[p] = oy;
-        if (wz) wz[p] = oz;
-        if (W) W[p] = on;
+    };
+    const int pstart = i0 - 2;
+    const int nstep = (iend - i0) + 4;
+    load_plane(pstart);
+    for (int s = 0; s < nstep; ++s) {
+        const int p = pstart + s;
+        const int s0 = s % 3, s1 = (s + 2) % 3, s2 = (s + 1) % 3;  // ring slots of planes p, p-1, p-2
+        // (1) U of plane p into the ring, then the loads of plane p+1
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+            const int e = 4 * r + w;
+            if (e < TE) sm.u[s0][e][lane] = un[r];
+        }
+        if (s + 1 < nstep) load_plane(p + 1);
+        __syncthreads();
+        // (2) velocity of plane q = p-1 at rows 1 .. TE-2, lanes 1 .. 62
+        const int q = p - 1;
+        if (q >= 0 && q < g.L && s >= 1) {
+#pragma unroll
+            for (int r = 0; r < RW; ++r) {
+                const int e = 4 * r + w;
+                if (e < 1 || e > TE - 2) continue;
+                const int j = j0 - 2 + e;
+                double vx = 0, vy = 0, vz = 0;
+                if (lane >= 1 && lane <= 62 && kin && j >= 0 && j < g.N) {
+                    int kx, ky, kz;
+                    double gxr = 0, gxi = 0, gyr = 0, gyi = 0, gzr = 0, gzi = 0;
+                    const double2 c = sm.u[s1][e][lane];
+                    if (kinds3(q, j, k, g, kx, ky, kz)) {
+                        // slots: plane q+1 -> s0, q -> s1, q-1 -> s2
+                        const double2 xp = sm.u[s0][e][lane], xm = sm.u[s2][e][lane];
+                        const double2 yp = sm.u[s1][e + 1][lane], ym = sm.u[s1][e - 1][lane];
+                        const double2 zp = sm.u[s1][e][lane + 1], zm = sm.u[s1][e][lane - 1];
+                        auto d = [&](int kind, double ap, double a0, double am) {
+                            if (kind == D_FWD) return (lh2 * 2) * (ap - a0);
+                            if (kind == D_BWD) return (lh2 * 2) * (a0 - am);
+                            return lh2 * (ap - am);
+                        };
+                        gxr = d(kx, xp.x, c.x, xm.x);
+                        gxi = d(kx, xp.y, c.y, xm.y);
+                        gyr = d(ky, yp.x, c.x, ym.x);
+                        gyi = d(ky, yp.y, c.y, ym.y);
+                        gzr = d(kz, zp.x, c.x, zm.x);
+                        gzi = d(kz, zp.y, c.y, zm.y);
+                    }
+                    const double l_U2 = 1 / (2 * c.x * c.x + 2 * c.y * c.y + veps);
+                    vx = (c.x * gxi - c.y * gxr) * l_U2;
+                    vy = (c.x * gyi - c.y * gyr) * l_U2;
+                    vz = (c.x * gzi - c.y * gzr) * l_U2;
+                }
+                const int vs = (s + 2) % 3;  // velocity ring slot of plane q
+                sm.v[vs][0][e - 1][lane] = vx;
+                sm.v[vs][1][e - 1][lane] = vy;
+                sm.v[vs][2][e - 1][lane] = vz;
+            }
+        }
+        __syncthreads();
+        // (3) curl of plane c = p-2 at rows 2 .. TE-3, lanes 2 .. 61
+        const int cpl = p - 2;
+        if (cpl >= i0 && cpl < iend) {
+            const int vp = (s + 2) % 3, vc = (s + 1) % 3, vm1 = s % 3;  // velocity of planes c+1, c, c-1
+#pragma unroll
+            for (int r = 0; r < RW; ++r) {
+                const int e = 4 * r + w;
+                if (e < 2 || e > TE - 3) continue;
+                const int j = j0 - 2 + e;
+                if (lane < 2 || lane > 61 || !kin || j >= g.N) continue;
+                int kx, ky, kz;
+                double ox = 0, oy = 0, oz = 0, on = 0;
+                if (kinds3(cpl, j, k, g, kx, ky, kz)) {
+                    auto d = [&](int kind, int comp, int axis) {
+                        // A(+1), A(0), A(-1) along the axis: 0 = x (planes), 1 = y (rows), 2 = z (lanes)
+                        double ap, a0 = sm.v[vc][comp][e - 1][lane], am;
+                        if (axis == 0) {
+                            ap = sm.v[vp][comp][e - 1][lane];
+                            am = sm.v[vm1][comp][e - 1][lane];
+                        } else if (axis == 1) {
+                            ap = sm.v[vc][comp][e][lane];
+                            am = sm.v[vc][comp][e - 2][lane];
+                        } else {
+                            ap = sm.v[vc][comp][e - 1][lane + 1];
+                            am = sm.v[vc][comp][e - 1][lane - 1];
+                        }
+                        if (kind == D_FWD) return (lh2 * 2) * (ap - a0);
+                        if (kind == D_BWD) return (lh2 * 2) * (a0 - am);
+                        return lh2 * (ap - am);
+                    };
+                    const double vzx = d(kx, 2, 0), vyx = d(kx, 1, 0);
+                    const double vzy = d(ky, 2, 1), vxy = d(ky, 0, 1);
+                    const double vyz = d(kz, 1, 2), vxz = d(kz, 0, 2);
+                    ox = vzy - vyz;
+                    oy = vxz - vzx;
+                    oz = vyx - vxy;
+                    on = sqrt(ox * ox + oy * oy + oz * oz);
+                }
+                const long pp = (long)cpl * g.si + (long)j * g.sj + k;
+                if (wx) wx[pp] = ox;
+                if (wy) wy[pp] = oy;
+                if (wz) wz[pp] = oz;
+                if (W) W[pp] = on;
+            }
+        }
     }
 }
 
 }  // namespace
 
-// V into three scratch planes of n doubles, then the curl into the outputs
-// (device pointers; any output may be null).  Grids need >= 3 points along
-// every axis (checked by the callers).
+// the vorticity into wx, wy, wz, W (device pointers; any output may be
+// null).  Grids need >= 3 points along every axis (checked by the callers).
 template <typename T>
 hipError_t launch_vorticity3d(const c2<T>* u, const c2<T>* ulo, int L, int N, int M, double lh2, double veps,
-                              double* vx, double* vy, double* vz, double* wx, double* wy, double* wz, double* W,
-                              hipStream_t st)
+                              double* wx, double* wy, double* wz, double* W, hipStream_t st)
 {
+    constexpr int TJ = 4;
     Dims3 g;
     g.L = L;
     g.N = N;
     g.M = M;
     g.sj = M;
     g.si = (long)N * M;
-    if (L > 65535 || (N + VT_J - 1) / VT_J > 65535) return hipErrorInvalidValue;  // grid y, z limits
-    const dim3 grid((M + VT_K - 1) / VT_K, (N + VT_J - 1) / VT_J, L), block(VT_K, VT_J);
-    hipLaunchKernelGGL((vel3d_kernel<T>), grid, block, 0, st, u, ulo, lh2, veps, g, vx, vy, vz);
-    hipLaunchKernelGGL(curl3d_kernel, grid, block, 0, st, vx, vy, vz, lh2, g, wx, wy, wz, W);
+    VortMarch vm;
+    vm.IC = 32;
+    vm.tj = (N + TJ - 1) / TJ;
+    vm.tk = (M + 59) / 60;
+    vm.ti = (L + vm.IC - 1) / vm.IC;
+    const long tiles = (long)vm.tj * vm.tk * vm.ti;
+    if (tiles > (1L << 30)) return hipErrorInvalidValue;
+    vm.per_xcd = (int)((tiles + 7) / 8);
+    hipLaunchKernelGGL((vort_march_kernel<T, TJ>), dim3(8 * vm.per_xcd), dim3(256), 0, st, u, ulo, lh2, veps, g, vm,
+                       wx, wy, wz, W);
     return hipGetLastError();
 }
 
 template hipError_t launch_vorticity3d<double>(const double2*, const double2*, int, int, int, double, double, double*,
-                                               double*, double*, double*, double*, double*, double*, hipStream_t);
+                                               double*, double*, double*, hipStream_t);
 template hipError_t launch_vorticity3d<float>(const float2*, const float2*, int, int, int, double, double, double*,
-                                              double*, double*, double*, double*, double*, double*, hipStream_t);
+                                              double*, double*, double*, hipStream_t);
 
 }  // namespace nlse
