@@ -209,6 +209,76 @@ def run_batched(args):
     print(json.dumps(line))
 
 
+def run_2d_matrix(args):
+    """The 2D configuration matrix on the headline grid (1024^2, MSD, the
+    headline's initial state): {CD, 2SHOC} x {FP64, FP32, compensated FP32}
+    at k = 0.3 h^2 (CD, the reference's stated step) and 0.2121 h^2 (2SHOC,
+    the driver's rule), 2 warm-up and 20 timed chunks of 50 steps each (device
+    time, HIP events); per variant the mean launch time, the fraction of HBM
+    peak on the bytes its fused kernel moves, and the diagnostics.  Also 2SHOC
+    at k = 0.3 h^2, beyond RK4's stability bound: the first step at which the
+    state stops being finite.  CPU: the reference integrators (1 thread) on a
+    bounded sample of each method."""
+    import nlsemagic_amd as nm
+    from oracle import oracle as O
+    U, V = initial_state()
+    res = {}
+    for m, kh2 in ((1, 0.3), (2, K_RULE)):
+        for prec in (2, 1, 3):
+            sol = nm.Solver((N, M), m, prec, 2, -1.0, 1.0, H * H, kh2 * H * H)
+            sol.set_graph(CHUNK)
+            sol.set_state(U, V)
+            sol.step(2 * CHUNK)
+            sol.sync()
+            ms = sol.step_timed(20 * CHUNK)
+            fz, kind, rows = sol.config()
+            launches = sol.profile_launches(10)
+            t_launch = float(np.mean(launches)) * 1e-3
+            mass, maxre, bad = sol.diagnostics()
+            dbytes = design_bytes(prec, fz)
+            name = f"{'CD' if m == 1 else '2SHOC'}_{ {2: 'fp64', 1: 'fp32', 3: 'fp32c'}[prec] }"
+            res[name] = {"rate": N * M * 20 * CHUNK / (ms * 1e-3), "k_over_h2": round(kh2, 6),
+                         "launch_us": round(t_launch * 1e6, 2), "stages_per_launch": fz, "kernel_kind": kind,
+                         "rows": rows, "design_bytes_per_point_step": dbytes,
+                         "design_frac": round(dbytes * N * M / (t_launch * (4 // fz)) / 1e9 / HBM_PEAK_GBS, 4),
+                         "mass_over_initial": mass / float(np.sum(np.abs(U) ** 2)), "nonfinite": int(bad)}
+            sol.close()
+    # 2SHOC at the stated k = 0.3 h^2: RK4 is unstable there for this stencil
+    sol = nm.Solver((N, M), 2, 2, 2, -1.0, 1.0, H * H, 0.3 * H * H)
+    sol.set_state(U, V)
+    blow = None
+    for c in range(200):
+        sol.step(10)
+        if sol.diagnostics()[2] > 0:
+            blow = 10 * (c + 1)
+            break
+    sol.close()
+    cpu = {}
+    use_ref = O.ref(2) is not None
+    for m, kh2 in ((1, 0.3), (2, K_RULE)):
+        t0 = time.perf_counter()
+        steps = 0
+        while time.perf_counter() - t0 < 5.0:
+            O.take_steps(U, V, -1.0, 1.0, H * H, 2, 10, kh2 * H * H, m, use_ref=use_ref)
+            steps += 10
+        cpu["CD" if m == 1 else "2SHOC"] = N * M * steps / (time.perf_counter() - t0)
+    head = res["2SHOC_fp64"]
+    line = {
+        "metric": "RK4 grid-point updates/s & % HBM peak; batched-solve p50/p99 latency",
+        "value": head["rate"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": 20, "warmup": 2,
+        "ms_per_step": N * M * CHUNK / head["rate"] * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64 (value); f64/f32/f32c (matrix)", "data": "synthetic",
+        "config": {"workload": "2D RK4 1024x1024, MSD, dark vortex + 1e-3 noise (seed 0): CD and 2SHOC x FP64/FP32/"
+                               "FP32c, chunks of 50 steps",
+                   "matrix": res,
+                   "twoshoc_k0.3h2_first_nonfinite_step": blow,
+                   "cpu_rate_1thread": cpu, "cpu_kind": "reference" if use_ref else "port",
+                   "speedup_vs_cpu": {k.split("_")[0] + "_fp64": res[k]["rate"] / cpu[k.split("_")[0]]
+                                      for k in ("CD_fp64", "2SHOC_fp64")}},
+    }
+    print(json.dumps(line))
+
+
 def vortex_rings_3d(n=256, L=25.6, nrings=4, radius=8.0, seed=0):
     """Product of dark vortex rings tanh(d/sqrt 2) e^{+-i phi} on [-L, L)^3:
     d is the distance to the ring core, phi the poloidal angle around it.
@@ -401,7 +471,7 @@ def main():
                     help="batched: stages per launch, kernel kind, rows, grid (skips the tuner)")
     ap.add_argument("--qps", type=float, default=4.0, help="serving: Poisson arrival rate")
     ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
-    ap.add_argument("--workload", default="2d", choices=("2d", "batched", "1d", "3d", "serving"),
+    ap.add_argument("--workload", default="2d", choices=("2d", "2d-matrix", "batched", "1d", "3d", "serving"),
                     help="2d: the 1024^2 FP64 2SHOC headline; batched: 512 uneven problems, FP32c vs FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -421,6 +491,9 @@ def main():
         return
     if args.workload in ("1d", "3d"):
         run_grid(args)
+        return
+    if args.workload == "2d-matrix":
+        run_2d_matrix(args)
         return
     if args.workload == "batched":
         if args.steps == ap.get_default("steps"):
