@@ -130,8 +130,8 @@ __global__ __launch_bounds__(256) void f3_kernel(const c2<T>* __restrict__ y, co
 
 // 2SHOC stage with D on chip: one workgroup per (j tile, k tile, i chunk);
 // consecutive tiles of a chunk go to the same XCD
-template <typename T, int TJ>
-__global__ __launch_bounds__(256) void march3d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+template <typename T, int TJ, int NW>
+__global__ __launch_bounds__(NW * 64) void march3d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
                                                       c2<T>* __restrict__ acc, c2<T>* yout, const T* __restrict__ V,
                                                       Coefs<T> c, March3 g, int BC, int stage, c2<T>* ulo)
 {
@@ -140,10 +140,11 @@ __global__ __launch_bounds__(256) void march3d_kernel(const c2<T>* __restrict__ 
     if (t >= g.tj * g.tk * g.ti) return;
     const int kt = t % g.tk, rest = t / g.tk;
     const int jt = rest % g.tj, ci = rest / g.tj;
-    march3d_tile<T, TJ>(sm, ci * g.IC, jt * TJ, kt * March3Cfg<TJ>::W, g, yin, u, acc, yout, V, c, BC, stage, ulo);
+    march3d_tile<T, TJ, NW>(sm, ci * g.IC, jt * TJ, kt * March3Cfg<TJ>::W, g, yin, u, acc, yout, V, c, BC, stage,
+                            ulo);
 }
 
-template <typename T, int TJ>
+template <typename T, int TJ, int NW>
 static hipError_t launch_march3d(const StageArgs<T>& a, int IC, hipStream_t st)
 {
     March3 g;
@@ -155,8 +156,8 @@ static hipError_t launch_march3d(const StageArgs<T>& a, int IC, hipStream_t st)
     g.tk = (a.M + March3Cfg<TJ>::W - 1) / March3Cfg<TJ>::W;
     g.ti = (a.L + IC - 1) / IC;
     g.per_xcd = (g.tj * g.tk * g.ti + 7) / 8;
-    hipLaunchKernelGGL((march3d_kernel<T, TJ>), dim3(8 * g.per_xcd), dim3(256), 0, st, a.yin, a.u, a.acc, a.yout,
-                       a.V, a.c, g, a.BC, a.stage, a.ulo);
+    hipLaunchKernelGGL((march3d_kernel<T, TJ, NW>), dim3(8 * g.per_xcd), dim3(NW * 64), 0, st, a.yin, a.u, a.acc,
+                       a.yout, a.V, a.c, g, a.BC, a.stage, a.ulo);
     return hipGetLastError();
 }
 
@@ -165,8 +166,8 @@ hipError_t launch_stage3d(const StageArgs<T>& a, hipStream_t st)
 {
     if (a.method == 2 && a.BC != BC_MSD && a.kernel != KERNEL_TILES) {
         // rows per tile: 4 or 8 (a.tile below 8 selects 4); planes per chunk 32
-        if (a.tile < 8) return launch_march3d<T, 4>(a, 32, st);
-        return launch_march3d<T, 8>(a, 32, st);
+        if (a.tile < 8) return launch_march3d<T, 4, 4>(a, 32, st);
+        return launch_march3d<T, 8, 4>(a, 32, st);
     }
     Grid3 g;
     g.N = a.N;

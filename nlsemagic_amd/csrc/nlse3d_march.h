@@ -1,9 +1,9 @@
 // nlse3d_march.h -- 3D 2SHOC RK4 stage with D kept on chip (device code).
 //
-// A workgroup of four waves owns a tile of TJ rows (j) x 60 columns (k) and
+// A workgroup of NW waves owns a tile of TJ rows (j) x 60 columns (k) and
 // marches along i over a chunk of IC planes:
 //   - lane l is column k0 - 2 + l (two halo columns each side), extended row
-//     e = 0 .. TJ+3 is row j0 - 2 + e and belongs to wave e mod 4;
+//     e = 0 .. TJ+3 is row j0 - 2 + e and belongs to wave e mod NW;
 //   - at step p the plane p of the stage input arrives (one coalesced row
 //     load per row), D = Lap_CD(Y)/h^2 of plane p-1 is formed, and F plus
 //     the RK4 update of plane p-2 (the output plane) follow;
@@ -33,10 +33,10 @@ struct March3 {
     int per_xcd;     // workgroups per XCD
 };
 
-template <int TJ>
+template <int TJ, int NW = 4>
 struct March3Cfg {
-    static constexpr int TE = TJ + 4;        // extended rows
-    static constexpr int RW = (TE + 3) / 4;  // extended rows per wave
+    static constexpr int TE = TJ + 4;               // extended rows
+    static constexpr int RW = (TE + NW - 1) / NW;   // extended rows per wave
     static constexpr int W = 60;             // owned columns
 };
 
@@ -47,17 +47,17 @@ struct March3Lds {
     c2<T> d[2][TE * 64];
 };
 
-template <typename T, int TJ>
+template <typename T, int TJ, int NW>
 __device__ __forceinline__ void march3d_tile(March3Lds<T, TJ>& sm, int i0, int j0, int k0, const March3& g,
                                              const c2<T>* __restrict__ yin, const c2<T>* u, c2<T>* __restrict__ acc,
                                              c2<T>* yout, const T* __restrict__ V, const Coefs<T>& c, int BC,
                                              int stage, c2<T>* ulo)
 {
-    using Cfg = March3Cfg<TJ>;
+    using Cfg = March3Cfg<TJ, NW>;
     constexpr int TE = Cfg::TE, RW = Cfg::RW;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    __builtin_assume(w >= 0 && w < 4);
+    __builtin_assume(w >= 0 && w < NW);
     const int N = g.N, M = g.M, L = g.L;
     const int k = k0 - 2 + lane;
     const int kc = k < 0 ? 0 : (k >= M ? M - 1 : k);
@@ -70,7 +70,7 @@ __device__ __forceinline__ void march3d_tile(March3Lds<T, TJ>& sm, int i0, int j
     long rowoff[RW];
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
-        const int e = 4 * r + w;
+        const int e = NW * r + w;
         const int j = j0 - 2 + e;
         jrow[r] = j;
         rowoff[r] = (long)(j < 0 ? 0 : (j >= N ? N - 1 : j)) * M + kc;
@@ -114,7 +114,7 @@ __device__ __forceinline__ void march3d_tile(March3Lds<T, TJ>& sm, int i0, int j
             // (a) plane p of the stage input, then the loads of plane p+1
 #pragma unroll
             for (int r = 0; r < RW; ++r) {
-                const int e = 4 * r + w;
+                const int e = NW * r + w;
                 Yr[s0][r] = Yn[r];
                 Vr[s0][r] = Vn[r];
                 if (e < TE) sm.y[s0][1 + e * 64 + lane] = Yr[s0][r];
@@ -125,7 +125,7 @@ __device__ __forceinline__ void march3d_tile(March3Lds<T, TJ>& sm, int i0, int j
             if (pd >= 0 && pd < L) {
 #pragma unroll
                 for (int r = 0; r < RW; ++r) {
-                    const int e = 4 * r + w;
+                    const int e = NW * r + w;
                     if (e < 1 || e > TE - 2) continue;
                     const int j = jrow[r];
                     const c2<T> m = Yr[s1][r];
@@ -149,7 +149,7 @@ __device__ __forceinline__ void march3d_tile(March3Lds<T, TJ>& sm, int i0, int j
                 const int sd = (pf - pstart) & 1;
 #pragma unroll
                 for (int r = 0; r < RW; ++r) {
-                    const int e = 4 * r + w;
+                    const int e = NW * r + w;
                     if (e < 2 || e > TE - 3) continue;
                     const int j = jrow[r];
                     if (j >= N) continue;

@@ -252,3 +252,19 @@ def test_serving_default_ics_are_the_generators():
                                     0.25 * h * shape[2])
         print("CASE serving ic", shape, flush=True)
         assert np.max(np.abs(got - exp)) < 1e-11
+
+
+@pytest.mark.parametrize("turns", list(range(8)))
+def test_quarter_turn_rings_on_a_non_cubic_grid(turns):
+    """Every combination of quarter turns (the reference's permute + flip
+    path) on a grid with three different extents, against the oracle."""
+    th = [math.pi / 2 if turns & (1 << b) else 0.0 for b in range(3)]
+    shape, h, origin = (20, 24, 28), 0.3, (-3.45, -4.05, -2.85)
+    print("CASE quarter turns", th, flush=True)
+    S = _solver(shape, 2)
+    S.fill(1.0)
+    S.add_vortex_ring(1, -1.0, 0.2, -0.1, 0.3, *th, 0.0, 0.1, 0.0, 0.2, h, 1.0, -1.0, origin, 2.0)
+    exp = I.add_vortex_ring(1, -1.0, 0.2, -0.1, 0.3, *th, 0.0, 0.1, 0.0, 0.2, np.ones(shape, complex), h, 1.0, -1.0,
+                            origin, 2.0)
+    assert np.max(np.abs(S.state() - exp)) <= 1e-11
+    S.close()

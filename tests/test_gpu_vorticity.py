@@ -36,7 +36,9 @@ def _edges(shape):
     return nb >= 2
 
 
-@pytest.mark.parametrize("shape", [(3, 3, 3), (3, 4, 5), (5, 7, 9), (12, 17, 33), (9, 64, 65), (33, 20, 130)])
+@pytest.mark.parametrize("shape", [(3, 3, 3), (3, 4, 5), (5, 7, 9), (12, 17, 33), (9, 64, 65), (33, 20, 130),
+                                   # tile edges: 16 rows, 60 columns, chunks of 16 planes
+                                   (16, 16, 60), (17, 17, 61), (15, 33, 59), (32, 15, 121), (48, 31, 120)])
 @pytest.mark.parametrize("veps", [1e-3, 0.0])
 def test_drop_in_matches_oracle(oracle, shape, veps):
     import nlsemagic_amd as nm
