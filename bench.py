@@ -327,6 +327,74 @@ def time_vorticity(sol, h, veps=1e-3, reps=20):
     return dict(ms=dt * 1e3, rate=n / dt, gbs=VORT_BYTES * n / dt / 1e9, host=sol.vorticity(1 / (2 * h), veps))
 
 
+PROFILE_VORT_TRAFFIC = os.path.join(ROOT, "profiles", "vorticity3d_256_traffic.json")
+
+
+def run_vorticity(args):
+    """3D vorticity (rmc_vorticity3D) of a 256^3 FP64 state: four dark
+    vortex rings of radius 8 made on the device by the ring generator
+    (NLSE_add_vortex_ring, opt_2D = 0) on the unit background, orientations,
+    centres and charges from seed 0 (per ring: thetax and thetay uniform in
+    [0, pi), the centre uniform in [-6, 6]^3, the charge +-1), then the
+    vorticity (lh2 = 1/(2h), veps = 1e-3) into device arrays: 5 untimed and
+    `steps` timed calls between HIP events on the solver's stream.  Checked
+    bit for bit against the oracle; CPU: the oracle's C restatement, 1
+    thread (the reference's own file needs MATLAB's mex.h)."""
+    import nlsemagic_amd as nm
+    from oracle import oracle as O
+    n, h = 256, 0.2
+    shape = (n, n, n)
+    origin = (-25.6, -25.6, -25.6)
+    sol = nm.Solver(shape, 2, 2, 1, -1.0, 1.0, h * h, 0.1 * h * h)
+    rng = np.random.default_rng(0)
+    t0 = time.perf_counter()
+    sol.fill(1.0)
+    for _ in range(4):
+        tx, ty = rng.uniform(0, np.pi, 2)
+        c = rng.uniform(-6, 6, 3)
+        m = int(rng.choice([-1, 1]))
+        sol.add_vortex_ring(m, -1.0, *c, tx, ty, 0.0, 0.0, 0.0, 0.0, 0.0, h, 1.0, -1.0, origin, 8.0)
+    ic_s = time.perf_counter() - t0
+    npts = n ** 3
+    outs = [nm.DeviceArray(npts) for _ in range(4)]
+    ptrs = [d.address for d in outs]
+    lh2 = 1 / (2 * h)
+    sol.vorticity_timed(lh2, 1e-3, 5, *ptrs)
+    ms = sol.vorticity_timed(lh2, 1e-3, args.steps, *ptrs)
+    got = [d.numpy().reshape(shape) for d in outs]
+    for d in outs:
+        d.free()
+    U = sol.state()
+    sol.close()
+    t0 = time.perf_counter()
+    exp = O.vorticity3d(U, lh2, 1e-3)
+    cpu_s = time.perf_counter() - t0
+    same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(got, exp))
+    alg, need = 96, 48
+    achieved = alg * npts / (ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PROFILE_VORT_TRAFFIC):
+        with open(PROFILE_VORT_TRAFFIC) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": "RK4 grid-point updates/s & % HBM peak; batched-solve p50/p99 latency",
+        "value": npts / (ms * 1e-3), "unit": "vorticity points/s", "n_gpus": 1, "steps": args.steps, "warmup": 5,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "3D vorticity (rmc_vorticity3D), 256^3 FP64, four dark vortex rings made on the "
+                               "device (seed 0)",
+                   "ic_four_rings_s": ic_s, "bit_identical_to_oracle": bool(same)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "vort_march_kernel<double,16,16>", "launch_us": round(ms * 1e3, 2),
+                     "algorithmic_bytes_per_point": alg, "design_bytes_per_point": need,
+                     "design_frac": round(need * npts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": {"value": npts / cpu_s, "unit": "vorticity points/s", "cores": 1, "kind": "port",
+                         "sample": f"the whole 256^3 state once ({cpu_s:.2f} s, oracle C restatement)"},
+    }
+    print(json.dumps(line))
+
+
 def run_grid(args):
     """1D (N = 2048, CD, MSD, dark soliton) or 3D (256^3, Dirichlet, vortex
     rings) single-problem workloads: device rate for FP64 / FP32 / FP32c and
@@ -471,7 +539,8 @@ def main():
                     help="batched: stages per launch, kernel kind, rows, grid (skips the tuner)")
     ap.add_argument("--qps", type=float, default=4.0, help="serving: Poisson arrival rate")
     ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
-    ap.add_argument("--workload", default="2d", choices=("2d", "2d-matrix", "batched", "1d", "3d", "serving"),
+    ap.add_argument("--workload", default="2d",
+                    choices=("2d", "2d-matrix", "batched", "1d", "3d", "serving", "vorticity"),
                     help="2d: the 1024^2 FP64 2SHOC headline; batched: 512 uneven problems, FP32c vs FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -494,6 +563,11 @@ def main():
         return
     if args.workload == "2d-matrix":
         run_2d_matrix(args)
+        return
+    if args.workload == "vorticity":
+        if args.steps == ap.get_default("steps"):
+            args.steps = 20
+        run_vorticity(args)
         return
     if args.workload == "batched":
         if args.steps == ap.get_default("steps"):

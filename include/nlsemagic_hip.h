@@ -133,6 +133,10 @@ int nlse_vorticity(nlse_solver *h, double lh2, double veps, double *wx, double *
  * on the solver's stream without allocating (one kernel, no scratch) */
 int nlse_vorticity_device(nlse_solver *h, double lh2, double veps, double *wx, double *wy, double *wz,
                           double *W);
+/* measurement helper: `reps` nlse_vorticity_device calls between two HIP
+ * events on the solver's stream; mean device ms per call */
+int nlse_vorticity_timed(nlse_solver *h, double lh2, double veps, double *wx, double *wy, double *wz,
+                         double *W, int reps, double *ms);
 /* ---- closed-form fields and initial conditions of the resident state ----
  * Grid coordinates follow the reference drivers' meshgrid: 1D x = xmin + i h;
  * 2D U[i*M + j] at x = xmin + i h, y = ymin + j h; 3D U[(i*N + j)*M + k] at

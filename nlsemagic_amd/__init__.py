@@ -131,6 +131,9 @@ def lib():
     L.NLSE3D_VORTICITY_HIP.argtypes = [_D, _D, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int] + [_D] * 4
     L.nlse_vorticity.restype = C.c_int
     L.nlse_vorticity.argtypes = [C.c_void_p, C.c_double, C.c_double] + [_D] * 4
+    L.nlse_vorticity_timed.restype = C.c_int
+    L.nlse_vorticity_timed.argtypes = ([C.c_void_p, C.c_double, C.c_double] + [C.c_void_p] * 4 +
+                                       [C.c_int, C.POINTER(C.c_double)])
     L.nlse_vorticity_device.restype = C.c_int
     L.nlse_vorticity_device.argtypes = [C.c_void_p, C.c_double, C.c_double] + [C.c_void_p] * 4
     L.nlse_set_exact.restype = C.c_int
@@ -408,6 +411,15 @@ class Solver:
         _check(self._lib.nlse_add_vortex_ring(self._h, int(m), *map(float, (OM, x0, y0, z0, thetax, thetay, thetaz, vx,
                                                                            vy, vz, phase, h, a, s, *origin, d)),
                                               int(opt_2D), float(ea), float(eb)))
+
+    def vorticity_timed(self, lh2, veps, reps, wx=0, wy=0, wz=0, W=0):
+        """Mean device ms of `reps` vorticity_device calls (HIP events on
+        the solver's stream)."""
+        ms = C.c_double()
+        _check(self._lib.nlse_vorticity_timed(self._h, float(lh2), float(veps),
+                                              *[C.c_void_p(int(x) or None) for x in (wx, wy, wz, W)],
+                                              int(reps), C.byref(ms)))
+        return ms.value
 
     def profile_launches(self, steps):
         """Mean device ms of each kernel launch of an RK4 step (HIP events)."""

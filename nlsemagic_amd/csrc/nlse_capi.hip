@@ -592,6 +592,23 @@ int nlse_vorticity_device(nlse_solver* h, double lh2, double veps, double* wx, d
     return NLSE_OK;
 }
 
+int nlse_vorticity_timed(nlse_solver* h, double lh2, double veps, double* wx, double* wy, double* wz, double* W,
+                         int reps, double* ms)
+{
+    if (!h || !ms || reps < 1) return fail(NLSE_EINVAL, "null argument or reps < 1");
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int r = 0; r < reps; ++r) {
+        const int rc = nlse_vorticity_device(h, lh2, veps, wx, wy, wz, W);
+        if (rc != NLSE_OK) return rc;
+    }
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float f = 0.f;
+    HIPCHK(hipEventElapsedTime(&f, h->ev0, h->ev1));
+    *ms = (double)f / reps;
+    return NLSE_OK;
+}
+
 int nlse_vorticity(nlse_solver* h, double lh2, double veps, double* wx, double* wy, double* wz, double* W)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");
