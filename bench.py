@@ -100,7 +100,22 @@ def cpu_baseline(U, V, k, budget_s=12.0):
         os.sched_setaffinity(0, prev)
     return {"value": N * M * steps / dt, "unit": "grid-point-steps/s", "cores": 1,
             "kind": "reference" if use_ref else "port",
-            "sample": f"{steps} RK4 steps of the same 1024x1024 2SHOC FP64 problem ({dt:.1f} s, 1 thread)"}
+            "sample": f"{steps} RK4 steps of the same 1024x1024 2SHOC FP64 problem ({dt:.1f} s, 1 thread)"}, W, steps
+
+
+def parity_vs_cpu(U, V, k, W, steps, precision=2):
+    """The same `steps` steps from the same state on the GPU (drop-in
+    solver path), compared with the CPU reference's result W."""
+    import nlsemagic_amd as nm
+    sol = nm.Solver((N, M), 2, precision, 2, -1.0, 1.0, H * H, k)
+    sol.set_state(U, V)
+    sol.step(steps)
+    G = sol.state()
+    sol.close()
+    m0, mc, mg = (float(np.sum(np.abs(A) ** 2)) for A in (U, W, G))
+    return {"steps": steps, "bit_identical": bool(np.array_equal(G.view(np.uint64), W.view(np.uint64))),
+            "rel_l2_vs_cpu_reference": float(np.linalg.norm(G - W) / np.linalg.norm(W)),
+            "mass_drift_gpu": mg / m0 - 1, "mass_drift_cpu_reference": mc / m0 - 1}
 
 
 def batched_problems(nprob=512, seed=1):
@@ -629,9 +644,10 @@ def main():
                 "launch_us_event_per_kernel": [round(x * 1e3, 2) for x in sol.profile_launches(20)]}
     sol.close()
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu and args.precision == 2:
-        cpu = cpu_baseline(U, V, k)
+        cpu, W_cpu, cpu_steps = cpu_baseline(U, V, k)
+        parity = parity_vs_cpu(U, V, k, W_cpu, cpu_steps)
 
     line = {
         "metric": "RK4 grid-point updates/s & % HBM peak; batched-solve p50/p99 latency",
@@ -656,6 +672,7 @@ def main():
     }
     if cpu:
         line["config"]["speedup_vs_cpu_reference"] = value / cpu["value"]
+        line["config"]["parity_vs_cpu_reference"] = parity
     print(json.dumps(line))
 
 

@@ -127,6 +127,24 @@ def test_resident_device_outputs_match_host_outputs():
     S.close()
 
 
+def test_vorticity_timed_gives_the_same_outputs():
+    import nlsemagic_amd as nm
+    shape = (6, 20, 70)
+    n = int(np.prod(shape))
+    S = nm.Solver(shape, 1, 3, 1, -1.0, 1.0, 0.04, 0.004)
+    S.set_state(_state(shape, 9))
+    host = S.vorticity(2.0, 1e-3)
+    dev = [nm.DeviceArray(n) for _ in range(4)]
+    ms = S.vorticity_timed(2.0, 1e-3, 3, *[d.address for d in dev])
+    assert ms > 0
+    for d, hst in zip(dev, host):
+        assert bits_equal(d.numpy().reshape(shape), hst)
+        d.free()
+    with pytest.raises(nm.HipError, match="reps"):
+        S.vorticity_timed(2.0, 1e-3, 0)
+    S.close()
+
+
 def test_vorticity_rejects_non_3d_solvers():
     import nlsemagic_amd as nm
     S = nm.Solver((16, 16), 1, 2, 1, -1.0, 1.0, 0.04, 0.004)
