@@ -238,6 +238,7 @@ def run_2d_matrix(args):
     from oracle import oracle as O
     U, V = initial_state()
     res = {}
+    final = {}
     for m, kh2 in ((1, 0.3), (2, K_RULE)):
         for prec in (2, 1, 3):
             sol = nm.Solver((N, M), m, prec, 2, -1.0, 1.0, H * H, kh2 * H * H)
@@ -250,6 +251,7 @@ def run_2d_matrix(args):
             launches = sol.profile_launches(10)
             t_launch = float(np.mean(launches)) * 1e-3
             mass, maxre, bad = sol.diagnostics()
+            final[(m, prec)] = sol.state()
             dbytes = design_bytes(prec, fz)
             name = f"{'CD' if m == 1 else '2SHOC'}_{ {2: 'fp64', 1: 'fp32', 3: 'fp32c'}[prec] }"
             res[name] = {"rate": N * M * 20 * CHUNK / (ms * 1e-3), "k_over_h2": round(kh2, 6),
@@ -258,6 +260,14 @@ def run_2d_matrix(args):
                          "design_frac": round(dbytes * N * M / (t_launch * (4 // fz)) / 1e9 / HBM_PEAK_GBS, 4),
                          "mass_over_initial": mass / float(np.sum(np.abs(U) ** 2)), "nonfinite": int(bad)}
             sol.close()
+    # single precision against the FP64 state after the same 1100 steps (the
+    # FP64 state is bit-identical to the CPU reference's, as the tests show)
+    for m in (1, 2):
+        ref = final[(m, 2)]
+        for prec in (1, 3):
+            name = f"{'CD' if m == 1 else '2SHOC'}_{ {1: 'fp32', 3: 'fp32c'}[prec] }"
+            res[name]["rel_l2_vs_fp64_after_1100_steps"] = float(np.linalg.norm(final[(m, prec)] - ref) /
+                                                                 np.linalg.norm(ref))
     # 2SHOC at the stated k = 0.3 h^2: RK4 is unstable there for this stencil
     sol = nm.Solver((N, M), 2, 2, 2, -1.0, 1.0, H * H, 0.3 * H * H)
     sol.set_state(U, V)
@@ -455,11 +465,22 @@ def run_grid(args):
     # CPU integrator on a bounded sample (1 thread): the reference for 1D, the restatement for 3D
     use_ref = args.workload == "1d" and O.ref(1) is not None
     cpu = {}
+    full_size = {}
     for m in methods:
         steps = 2000 if args.workload == "1d" else 2
         t0 = time.perf_counter()
-        O.take_steps(U, V, -1.0, 1.0, h * h, BC, steps, ks[m], m, use_ref=use_ref)
+        Wc = O.take_steps(U, V, -1.0, 1.0, h * h, BC, steps, ks[m], m, use_ref=use_ref)
         cpu[m] = npts * steps / (time.perf_counter() - t0)
+        if args.workload == "3d":
+            # the same steps at full size on the GPU, FP64, against the CPU result
+            sol = nm.Solver(shape, m, 2, BC, -1.0, 1.0, h * h, ks[m])
+            sol.set_state(U, V)
+            sol.step(steps)
+            G = sol.state()
+            sol.close()
+            full_size["CD" if m == 1 else "2SHOC"] = {
+                "steps": steps, "bit_identical": bool(np.array_equal(G.view(np.uint64), Wc.view(np.uint64))),
+                "rel_l2": float(np.linalg.norm(G - Wc) / np.linalg.norm(Wc))}
     m0 = float(np.sum(np.abs(U) ** 2))
     fp64 = {m: out[(m, 2)] for m in methods}
     line = {
@@ -476,6 +497,7 @@ def run_grid(args):
                    "cpu_kind": "reference" if use_ref else "port"},
     }
     if args.workload == "3d":
+        line["config"]["fp64_vs_cpu_full_size"] = full_size
         t0 = time.perf_counter()
         exp = O.vorticity3d(vort["W"], 1 / (2 * h), 1e-3)
         cpu_v = npts / (time.perf_counter() - t0)
