@@ -217,7 +217,7 @@ int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
  * (default) or 8 rows x 60 columns; kind 0 runs a D pass and an F pass per
  * stage.  MSD boundaries always use the two passes.
  * 1D: kind 1 (default) advances `rows` steps (1..96 CD, 1..48 2SHOC; default
- * 16) per launch on 256-point windows with a halo, one per CU; kind 0 runs a
+ * 32) per launch on 256-point windows with a halo, one per CU; kind 0 runs a
  * whole chunk in one workgroup (N <= 4096) or one launch per stage. */
 int nlse_set_kernel(nlse_solver *h, int kind, int rows);
 /* nlse_set_kernel(h, 0, tile_rows) */

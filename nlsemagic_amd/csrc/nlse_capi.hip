@@ -451,7 +451,7 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
     h->ubase = h->U;
     h->fuse = (dim == 2) ? 2 : 1;
     h->kern = KERNEL_ROWS;         // the fastest family on a 1024^2 grid and for 3D 2SHOC
-    h->tile = (dim == 3) ? 4 : (dim == 1 ? 16 : 8);  // 3D: plane-march rows; 1D: steps per launch
+    h->tile = (dim == 3) ? 4 : (dim == 1 ? 32 : 8);  // 3D: plane-march rows; 1D: steps per launch
     set_params(h, BC, s, a, h2, k);
     // a defined initial state: zero field and potential
     (void)hipMemsetAsync(h->U, 0, eb, h->stream);
