@@ -22,6 +22,8 @@
 // the two-pass kernels and to the reference.
 #pragma once
 
+#include <type_traits>
+
 #include "nlse2d_strip.h"
 
 namespace nlse {
@@ -46,6 +48,35 @@ struct March3Lds {
     c2<T> y[4][TE * 64 + 2];  // one padding element before and after each plane
     c2<T> d[2][TE * 64];
 };
+
+// the RK4 stage update with the ktot and U_lo values already loaded
+// (the arithmetic of rk4_epilogue, operand for operand)
+template <typename T>
+__device__ __forceinline__ void rk4_update(int stage, const Coefs<T>& c, c2<T> f, c2<T> u, c2<T> a, c2<T> lo,
+                                           c2<T>* __restrict__ acc, c2<T>* yout, long p, c2<T>* ulo)
+{
+    if (stage == 1) {
+        acc[p] = f;
+        yout[p] = mk<T>(u.x + c.k2 * f.x, u.y + c.k2 * f.y);
+    } else if (stage == 4) {
+        a.x = a.x + (T)1.0 * f.x;
+        a.y = a.y + (T)1.0 * f.y;
+        if (ulo) {
+            comp_add<T>(u.x, lo.x, c.k6 * a.x);
+            comp_add<T>(u.y, lo.y, c.k6 * a.y);
+            ulo[p] = lo;
+            yout[p] = u;
+        } else {
+            yout[p] = mk<T>(u.x + c.k6 * a.x, u.y + c.k6 * a.y);
+        }
+    } else {
+        a.x = a.x + (T)2.0 * f.x;
+        a.y = a.y + (T)2.0 * f.y;
+        acc[p] = a;
+        const T kk = (stage == 2) ? c.k2 : c.k;
+        yout[p] = mk<T>(u.x + kk * f.x, u.y + kk * f.y);
+    }
+}
 
 template <typename T, int TJ, int NW>
 __device__ __forceinline__ void march3d_tile(March3Lds<T, TJ>& sm, int i0, int j0, int k0, const March3& g,
@@ -87,6 +118,15 @@ __device__ __forceinline__ void march3d_tile(March3Lds<T, TJ>& sm, int i0, int j
             Dr[s][r] = z;
             Vr[s][r] = 0;
         }
+    // RK4 update operands (U, ktot and, for a compensated single-precision
+    // state, U_lo) of the plane updated at the next step, loaded at the end
+    // of the step before
+    constexpr bool HAS_LO = std::is_same<T, float>::value;
+    c2<T> Up[RW], Ap[RW], Lp[HAS_LO ? RW : 1];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) Up[r] = Ap[r] = z;
+#pragma unroll
+    for (int r = 0; r < (HAS_LO ? RW : 1); ++r) Lp[r] = z;
     auto sy = [&](int slot, int e, int dl) -> c2<T> { return sm.y[slot][1 + e * 64 + lane + dl]; };
     auto sdr = [&](int slot, int e) -> c2<T> { return sm.d[slot][e * 64 + lane]; };
 
@@ -191,9 +231,26 @@ __device__ __forceinline__ void march3d_tile(March3Lds<T, TJ>& sm, int i0, int j
                     if (!inner || kb) f = (BC == BC_LAP0) ? lap0_f<T>(m, vx, c.s) : z;
                     if (own_k) {
                         const long pp = (long)pf * si + (long)j * M + k;
-                        const c2<T> uu = (stage == 1) ? m : u[pp];
-                        rk4_epilogue<T>(stage, c, f, uu, acc, yout, yout, pp, ulo);
+                        const c2<T> uu = (stage == 1) ? m : Up[r];
+                        rk4_update<T>(stage, c, f, uu, Ap[r], HAS_LO ? Lp[HAS_LO ? r : 0] : z, acc, yout, pp,
+                                      HAS_LO ? ulo : nullptr);
                     }
+                }
+            }
+            // (d) operands of plane p-1, updated at the next step (own rows;
+            // clamped addresses: values of columns this tile does not own are
+            // never used)
+            const int pn = p - 1;
+            if (stage != 1 && pn >= i0 && pn < iend) {
+#pragma unroll
+                for (int r = 0; r < RW; ++r) {
+                    const int e = NW * r + w;
+                    if (e < 2 || e > TE - 3) continue;
+                    const long pp = (long)pn * si + rowoff[r];
+                    Up[r] = u[pp];
+                    Ap[r] = acc[pp];
+                    if constexpr (HAS_LO)
+                        if (stage == 4 && ulo) Lp[r] = ulo[pp];
                 }
             }
             __syncthreads();  // the slots written at this step are complete; the oldest may be reused

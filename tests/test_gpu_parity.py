@@ -324,6 +324,26 @@ def test_march3d_matches_oracle_and_two_pass(hip, oracle, shape, prec, BC, rows,
     assert bits_equal(res[1], exp)
 
 
+@pytest.mark.parametrize("shape,BC", [((70, 20, 130), 1), ((33, 9, 61), 3), ((40, 17, 121), 0)])
+def test_march3d_compensated_matches_two_pass(hip, shape, BC):
+    """Compensated single precision (hi + lo state) on the plane-marching
+    kernel, whose update operands are loaded a plane ahead, equals the
+    two-pass kernels bit for bit after several steps."""
+    _announce("march3d comp", shape, BC)
+    rng = np.random.default_rng(sum(shape) + BC)
+    U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
+    V = 0.5 * rng.uniform(-1, 1, shape)
+    res = []
+    for kind in (1, 0):
+        sol = hip.Solver(shape, 2, 3, BC, -1.0, 0.9, 0.04, 0.004)
+        sol.set_kernel(kind, 4 if kind == 1 else 8)
+        sol.set_state(U, V)
+        sol.step(7)
+        res.append(sol.state())
+        sol.close()
+    assert bits_equal(res[0], res[1])
+
+
 def _block1d_cases():
     """1D time-blocked windows: grids smaller than, equal to and larger than
     one 256-point segment, steps per launch 1..max (CD 96, 2SHOC 48), chunk
