@@ -202,8 +202,10 @@ int nlse_vortex_profile(int m, double OM, double a, double s, double hr, int n, 
  * (nlaunch = 4 / stages per launch; per-kernel timing for roofline reports;
  * not available for 1D N <= 4096, which runs whole chunks in one kernel) */
 int nlse_profile_launches(nlse_solver *h, int steps, double *ms_launch, int *nlaunch);
-/* 2D: RK4 stages fused per kernel launch, 1, 2 or 4 (default 2; the results
- * are identical for every choice) */
+/* RK4 stages fused per kernel launch (the results are identical for every
+ * choice): 2D 1, 2 or 4 (default 2); 3D 1 or 2 (default 2, used by CD with a
+ * boundary other than MSD: two stages per plane-marching launch; 2SHOC and
+ * MSD run one stage per launch) */
 int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
 /* 2D: the stage kernel family and its rows per tile (results identical for
  * every choice):
@@ -228,7 +230,8 @@ int nlse_get_config(const nlse_solver *h, int *fusion, int *kind, int *rows);
 /* 2D: time `steps` steps for every configuration (stages per launch x kernel
  * kind x rows) on a saved copy of the state, keep the fastest, and restore
  * the state; reports the stages per launch, the rows and the ms per step
- * (any pointer may be NULL; nlse_get_config gives the kind).  1D/3D: no-op. */
+ * (any pointer may be NULL; nlse_get_config gives the kind).  1D/3D: no-op
+ * that reports the stages per launch in use. */
 int nlse_autotune(nlse_solver *h, int steps, int *best_fusion, int *best_tile, double *best_ms);
 /* replay the chunk loop from a captured hipGraph of `steps_per_graph` steps
  * (0 = plain stream launches, the default) */

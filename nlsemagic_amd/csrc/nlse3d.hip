@@ -14,6 +14,7 @@
 // Neighbour reads of the per-point kernels go through the L1/L2 caches: a
 // wavefront covers 64 consecutive k, so every plane and row neighbour is a
 // coalesced line read.
+#include "nlse3d_cdpair.h"
 #include "nlse3d_march.h"
 #include "nlse_common.h"
 #include "nlse_launch.h"
@@ -160,6 +161,46 @@ static hipError_t launch_march3d(const StageArgs<T>& a, int IC, hipStream_t st)
                        a.yout, a.V, a.c, g, a.BC, a.stage, a.ulo);
     return hipGetLastError();
 }
+
+// CD, two stages per launch (stages 1-2 or 3-4) on plane-marching tiles
+template <typename T, int FIRST, int TJ, int NW>
+__global__ __launch_bounds__(NW * 64) void cdpair3d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                           c2<T>* __restrict__ acc, c2<T>* __restrict__ yout,
+                                                           const T* __restrict__ V, Coefs<T> c, March3 g, int BC,
+                                                           c2<T>* ulo)
+{
+    __shared__ CdPairLds<T, TJ> sm;
+    const int t = (blockIdx.x & 7) * g.per_xcd + (blockIdx.x >> 3);
+    if (t >= g.tj * g.tk * g.ti) return;
+    const int kt = t % g.tk, rest = t / g.tk;
+    const int jt = rest % g.tj, ci = rest / g.tj;
+    cdpair_tile<T, FIRST, TJ, NW>(sm, ci * g.IC, jt * TJ, kt * 60, g, yin, u, acc, yout, V, c, BC, ulo);
+}
+
+template <typename T>
+hipError_t launch_pair3d(const StageArgs<T>& a, bool first, hipStream_t st)
+{
+    constexpr int TJ = 4, NW = 4;
+    March3 g;
+    g.N = a.N;
+    g.M = a.M;
+    g.L = a.L;
+    g.IC = 32;
+    g.tj = (a.N + TJ - 1) / TJ;
+    g.tk = (a.M + 59) / 60;
+    g.ti = (a.L + g.IC - 1) / g.IC;
+    g.per_xcd = (g.tj * g.tk * g.ti + 7) / 8;
+    if (first)
+        hipLaunchKernelGGL((cdpair3d_kernel<T, 1, TJ, NW>), dim3(8 * g.per_xcd), dim3(NW * 64), 0, st, a.yin, a.u,
+                           a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+    else
+        hipLaunchKernelGGL((cdpair3d_kernel<T, 3, TJ, NW>), dim3(8 * g.per_xcd), dim3(NW * 64), 0, st, a.yin, a.u,
+                           a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+    return hipGetLastError();
+}
+
+template hipError_t launch_pair3d<double>(const StageArgs<double>&, bool, hipStream_t);
+template hipError_t launch_pair3d<float>(const StageArgs<float>&, bool, hipStream_t);
 
 template <typename T>
 hipError_t launch_stage3d(const StageArgs<T>& a, hipStream_t st)

@@ -76,6 +76,8 @@ int build_tiles_t(nlse_batch* b)
         tp.insert(tp.end(), nt, p);
         t0 += nt;
     }
+    // steps already enqueued read the old tables: let them finish first
+    if (b->stream) HIPCHK(hipStreamSynchronize(b->stream));
     if (b->probs) (void)hipFree(b->probs);
     if (b->tile_prob) (void)hipFree(b->tile_prob);
     b->probs = nullptr;
@@ -245,9 +247,12 @@ int nlse_batch_create(nlse_batch** out, int nprob, const int* N, const int* M, c
         nlse_batch_destroy(b);
         return fail(NLSE_EHIP, "stream/event creation failed");
     }
-    (void)hipMemset(b->U, 0, eb);
-    (void)hipMemset(b->V, 0, eb / 2);
-    if (b->Ulo) (void)hipMemset(b->Ulo, 0, eb);
+    // cleared on the batch's own stream (a null-stream memset is not ordered
+    // with it and could land after the caller's first uploads)
+    (void)hipMemsetAsync(b->U, 0, eb, b->stream);
+    (void)hipMemsetAsync(b->V, 0, eb / 2, b->stream);
+    if (b->Ulo) (void)hipMemsetAsync(b->Ulo, 0, eb, b->stream);
+    (void)hipStreamSynchronize(b->stream);
     int rc = build_tiles(b);
     if (rc != NLSE_OK) {
         nlse_batch_destroy(b);

@@ -95,8 +95,14 @@ void drop_graphs(nlse_solver* h)
     }
 }
 
-// RK4 stages per launch actually used (temporal fusion is a 2D feature)
-int fusion(const nlse_solver* h) { return h->dim == 2 ? h->fuse : 1; }
+// RK4 stages per launch actually used: 2D 1, 2 or 4; 3D CD pairs of stages
+// (not MSD, whose partner may lie in another plane); 1 otherwise
+int fusion(const nlse_solver* h)
+{
+    if (h->dim == 2) return h->fuse;
+    if (h->dim == 3 && h->method == 1 && h->BC != BC_MSD && h->fuse == 2) return 2;
+    return 1;
+}
 
 void set_params(nlse_solver* h, int BC, double s, double a, double h2, double k)
 {
@@ -181,12 +187,12 @@ int enqueue_launch_t(nlse_solver* h, const Coefs<T>& c, int l)
     if (fz == 2 && l == 0) {
         a.yin = (const C*)h->U;
         a.yout = (C*)h->Y0;  // Y_2 (and ktot) after stages 1-2
-        e = launch_pair2d<T>(a, true, h->stream);
+        e = (h->dim == 3) ? launch_pair3d<T>(a, true, h->stream) : launch_pair2d<T>(a, true, h->stream);
         last = false;
     } else if (fz == 2) {
         a.yin = (const C*)h->Y0;
         a.yout = (C*)h->Y1;
-        e = launch_pair2d<T>(a, false, h->stream);
+        e = (h->dim == 3) ? launch_pair3d<T>(a, false, h->stream) : launch_pair2d<T>(a, false, h->stream);
         last = true;
     } else {
         a.yin = (const C*)h->U;
@@ -439,9 +445,9 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
         nlse_destroy(h);
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
-    if (precision == 3) {
-        if (hipMalloc(&h->Ulo, eb) != hipSuccess) { nlse_destroy(h); return fail(NLSE_ENOMEM, "hipMalloc failed"); }
-        (void)hipMemset(h->Ulo, 0, eb);
+    if (precision == 3 && hipMalloc(&h->Ulo, eb) != hipSuccess) {
+        nlse_destroy(h);
+        return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
@@ -449,13 +455,16 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
         return fail(NLSE_EHIP, "stream/event creation failed");
     }
     h->ubase = h->U;
-    h->fuse = (dim == 2) ? 2 : 1;
+    h->fuse = (dim >= 2) ? 2 : 1;  // 3D: CD pairs of stages (used where the method and BC allow)
     h->kern = KERNEL_ROWS;         // the fastest family on a 1024^2 grid and for 3D 2SHOC
     h->tile = (dim == 3) ? 4 : (dim == 1 ? 32 : 8);  // 3D: plane-march rows; 1D: steps per launch
     set_params(h, BC, s, a, h2, k);
-    // a defined initial state: zero field and potential
+    // a defined initial state: zero field and potential, cleared on the
+    // solver's own stream (a null-stream memset is not ordered with it and
+    // could land after the caller's first writes)
     (void)hipMemsetAsync(h->U, 0, eb, h->stream);
     (void)hipMemsetAsync(h->V, 0, eb / 2, h->stream);
+    if (h->Ulo) (void)hipMemsetAsync(h->Ulo, 0, eb, h->stream);
     (void)hipStreamSynchronize(h->stream);
     *out = h;
     return NLSE_OK;
@@ -723,7 +732,8 @@ int nlse_set_fusion(nlse_solver* h, int stages_per_launch)
     if (!h) return fail(NLSE_EINVAL, "null solver");
     if (stages_per_launch != 1 && stages_per_launch != 2 && stages_per_launch != 4)
         return fail(NLSE_EINVAL, "stages per launch must be 1, 2 or 4");
-    if (h->dim != 2 && stages_per_launch != 1) return fail(NLSE_EINVAL, "stage fusion is implemented for 2D");
+    if (h->dim == 1 && stages_per_launch != 1) return fail(NLSE_EINVAL, "stage fusion is implemented for 2D and 3D");
+    if (h->dim == 3 && stages_per_launch == 4) return fail(NLSE_EINVAL, "3D fuses at most two stages per launch");
     h->fuse = stages_per_launch;
     drop_graphs(h);
     return NLSE_OK;
@@ -774,8 +784,8 @@ int nlse_get_config(const nlse_solver* h, int* fusion_out, int* kind, int* rows)
 int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, double* best_ms)
 {
     if (!h || steps <= 0) return fail(NLSE_EINVAL, "bad argument");
-    if (h->dim != 2) {  // 1D and 3D have one kernel shape
-        if (best_fusion) *best_fusion = 1;
+    if (h->dim != 2) {  // 1D and 3D: nothing to tune; report what is in use
+        if (best_fusion) *best_fusion = fusion(h);
         if (best_tile) *best_tile = 0;
         if (best_ms) *best_ms = 0;
         return NLSE_OK;
