@@ -213,8 +213,7 @@ int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
  *   kind 1  row-sweep LDS tiles (one lane per column, DPP neighbours),
  *           8 (default), 12 or 16 rows
  *   kind 2  register-marching strips (one wave per strip, no LDS), segments
- *           of 3..256 rows, for 1 or 2 stages per launch (4 stages per launch
- *           run on kind 1 with 8 rows)
+ *           of 3..256 rows, for 1, 2 or 4 stages per launch
  * 3D 2SHOC: kind 1 (default) marches planes with D kept on chip, tiles of 4
  * (default) or 8 rows x 60 columns; kind 0 runs a D pass and an F pass per
  * stage.  MSD boundaries always use the two passes.
@@ -260,11 +259,12 @@ int nlse_batch_sync(nlse_batch *b);
 /* per-problem diagnostics, out[3p..3p+2] as for nlse_diagnostics */
 int nlse_batch_diagnostics(nlse_batch *b, double *out);
 /* stage kernel family: 0 LDS tiles (rows 8, 16, 32), 1 row-sweep tiles
- * (rows 8 (default), 12, 16); see nlse_set_kernel */
+ * (rows 8 (default), 12, 16), 2 register strips (3..256-row segments, one
+ * wave each); see nlse_set_kernel */
 int nlse_batch_set_kernel(nlse_batch *b, int kind, int rows);
 /* stages per launch, kernel kind, rows per tile, persistent grid */
 int nlse_batch_get_config(const nlse_batch *b, int *fusion, int *kind, int *rows, int *grid);
-/* stages per launch (1, 2, 4), rows per tile of the current kernel family,
+/* stages per launch (1, 2, 4), rows per tile (strip segment) of the current kernel family,
  * persistent grid (multiple of 8 workgroups) and hipGraph steps per replay
  * (0 = off) */
 int nlse_batch_configure(nlse_batch *b, int stages_per_launch, int tile_rows, int grid,

@@ -448,8 +448,7 @@ class Solver:
 
     def set_kernel(self, kind, rows):
         """2D stage kernel family and its rows per tile: 0 LDS tiles (8/16/32),
-        1 row-sweep tiles (8/12/16), 2 register strips (3..256; 1-2 stages per
-        launch)."""
+        1 row-sweep tiles (8/12/16), 2 register strips (3..256-row segments)."""
         _check(self._lib.nlse_set_kernel(self._h, int(kind), int(rows)))
 
     def config(self):
@@ -548,7 +547,7 @@ class Batch:
 
     def set_kernel(self, kind, rows):
         """Stage kernel family: 0 LDS tiles (8/16/32 rows), 1 row-sweep tiles
-        (8/12/16 rows)."""
+        (8/12/16 rows), 2 register strips (3..256-row segments)."""
         _check(self._lib.nlse_batch_set_kernel(self._h, int(kind), int(rows)))
 
     def config(self):

@@ -152,13 +152,7 @@ static hipError_t launch_rows(const StageArgs<T>& a, hipStream_t st)
 template <typename T, int METHOD, int FIRST, int LAST>
 static hipError_t launch_fused(const StageArgs<T>& a, hipStream_t st)
 {
-    if (a.kernel == KERNEL_STRIPS) {
-        if constexpr (LAST - FIRST < 2) return launch_strip<T, METHOD, FIRST, LAST>(a, st);
-        StageArgs<T> b = a;  // four stages per launch: row-sweep tiles
-        b.kernel = KERNEL_ROWS;
-        b.tile = 8;
-        return launch_rows<T, METHOD, FIRST, LAST>(b, st);
-    }
+    if (a.kernel == KERNEL_STRIPS) return launch_strip<T, METHOD, FIRST, LAST>(a, st);
     if (a.kernel == KERNEL_ROWS) return launch_rows<T, METHOD, FIRST, LAST>(a, st);
     switch (a.tile) {
     case 8: return launch_fused_ti<T, METHOD, 8, FIRST, LAST>(a, st);

@@ -1,9 +1,11 @@
 // nlse2d_batch.hip -- many independent 2D problems (uneven sizes, own s, a,
 // h, k and V each) advanced by ONE persistent launch per fused group of RK4
 // stages.  Each workgroup walks the concatenated tile list of all problems;
-// the tile code is the single-problem one (nlse2d_tile.h, nlse2d_rows.h), so
-// every problem's result is bit-identical to running it alone.
+// the tile code is the single-problem one (nlse2d_tile.h, nlse2d_rows.h,
+// nlse2d_strip.h), so every problem's result is bit-identical to running it
+// alone.
 #include "nlse2d_rows.h"
+#include "nlse2d_strip.h"
 #include "nlse2d_tile.h"
 
 namespace nlse {
@@ -67,6 +69,48 @@ __global__ __launch_bounds__(NW * 64) void batch2d_rows_kernel(const c2<T>* __re
     }
 }
 
+// the same walk over register-marching strips (nlse2d_strip.h): each wave
+// of a workgroup takes its own unit (a segment of S rows x one strip of
+// 64 - 2H columns of one problem); waves never synchronise
+template <typename T, int METHOD, int FIRST, int LAST, bool COMP>
+__global__ __launch_bounds__(256) void batch2d_strip_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                            c2<T>* __restrict__ acc, c2<T>* out,
+                                                            const T* __restrict__ V,
+                                                            const BatchProb2<T>* __restrict__ probs,
+                                                            const int* __restrict__ tile_prob, int ntiles, int S,
+                                                            c2<T>* __restrict__ ulo)
+{
+    constexpr int W = StripCfg<METHOD, FIRST, LAST>::W;
+    const int xcd = blockIdx.x & 7, nper = gridDim.x >> 3;
+    const int per = (ntiles + 7) >> 3;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = (blockIdx.x >> 3) * 4 + w; i < per; i += nper * 4) {
+        const int t = xcd * per + i;
+        if (t >= ntiles) break;
+        const BatchProb2<T>& P = probs[tile_prob[t]];
+        const int lt = t - P.tile0;
+        const int ti = lt / P.tiles_j, tj = lt - ti * P.tiles_j;
+        strip2d_wave<T, METHOD, FIRST, LAST, COMP>(ti * S - P.ioff, S, tj * W - P.joff, P.N, P.M, yin + P.off,
+                                                   u + P.off, acc + P.off, out + P.off, V + P.off, P.c, P.BC,
+                                                   COMP ? ulo + P.off : nullptr);
+    }
+}
+
+template <typename T, int METHOD, int FIRST, int LAST>
+static hipError_t launch_b_strips(const BatchArgs<T>& a, hipStream_t st)
+{
+    if constexpr (std::is_same<T, float>::value && LAST == 4) {
+        if (a.ulo) {
+            hipLaunchKernelGGL((batch2d_strip_kernel<T, METHOD, FIRST, LAST, true>), dim3(a.grid), dim3(256), 0, st,
+                               a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.tile, a.ulo);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((batch2d_strip_kernel<T, METHOD, FIRST, LAST, false>), dim3(a.grid), dim3(256), 0, st, a.yin,
+                       a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.tile, a.ulo);
+    return hipGetLastError();
+}
+
 template <typename T, int METHOD, int TI, int FIRST, int LAST>
 static hipError_t launch_b_rows(const BatchArgs<T>& a, hipStream_t st)
 {
@@ -95,6 +139,7 @@ static hipError_t launch_b(const BatchArgs<T>& a, hipStream_t st)
 template <typename T, int METHOD, int FIRST, int LAST>
 static hipError_t launch_b_t(const BatchArgs<T>& a, hipStream_t st)
 {
+    if (a.kernel == KERNEL_STRIPS) return launch_b_strips<T, METHOD, FIRST, LAST>(a, st);
     if (a.kernel == KERNEL_ROWS) {
         switch (a.tile) {
         case 12: return launch_b_rows<T, METHOD, 12, FIRST, LAST>(a, st);

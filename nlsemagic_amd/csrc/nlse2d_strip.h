@@ -10,9 +10,11 @@
 //   - for 2SHOC, D = Lap_CD(Y)/h^2 of row r-1 is formed when row r arrives,
 //     F of row r-2 one step later, and the RK4 epilogue of row r-3 after
 //     that (the extra row lets the top MSD row take its partner's F);
-//   - stage q+1 of a fused pair consumes the rows stage q emits in the same
-//     sweep, three (2SHOC) or two (CD) rows behind, so the intermediate Y
-//     never leaves the registers.
+//   - stage q+1 of a fused group (1, 2 or 4 stages) consumes the rows stage
+//     q emits in the same sweep, three (2SHOC) or two (CD) rows behind, so
+//     the intermediate Y never leaves the registers; each stage's rings are
+//     written three sweep steps before their last read and have four slots,
+//     whatever the number of stages.
 // Rows and columns outside the owned block are computed as halo (R per fused
 // stage); lanes whose inputs reach beyond the wave hold garbage that no owned
 // value depends on.  Boundary rows/columns follow the reference BC rules
@@ -63,7 +65,7 @@ struct StripCfg {
     static constexpr int HJ = NF * R;                 // input halo (columns and rows)
     static constexpr int W = 64 - 2 * HJ;             // owned columns per strip
     static constexpr int DLY = R + 1;                 // rows between a stage's input row and output row
-    static_assert(NF == 1 || NF == 2, "the 8-row register rings hold at most two fused stages");
+    static_assert(NF == 1 || NF == 2 || NF == 4, "one, two or four fused stages");
 };
 
 // One wave: RK4 stages FIRST..LAST on rows [i0, i0+S) x columns [j0, j0+W)

@@ -45,11 +45,27 @@ void drop_graphs(nlse_batch* b)
 }
 
 // tile width of the current kernel family: 64 columns, or for row-sweep tiles
-// 64 minus the halo of the fused stages on both sides
+// and strips 64 minus the halo of the fused stages on both sides
 int tile_width(const nlse_batch* b)
 {
-    if (b->kern != KERNEL_ROWS) return 64;
+    if (b->kern == KERNEL_TILES) return 64;
     return 64 - 2 * b->fuse * (b->method == 2 ? 2 : 1);
+}
+
+// rows per tile (strips: per segment) accepted by each kernel family
+int check_rows(int kind, int rows)
+{
+    if (kind == KERNEL_TILES) {
+        if (rows != 8 && rows != 16 && rows != 32) return fail(NLSE_EINVAL, "LDS tile rows must be 8, 16 or 32");
+    } else if (kind == KERNEL_ROWS) {
+        if (rows != 8 && rows != 12 && rows != 16) return fail(NLSE_EINVAL, "row-sweep tile rows must be 8, 12 or 16");
+    } else if (kind == KERNEL_STRIPS) {
+        if (rows < 3 || rows > 256) return fail(NLSE_EINVAL, "strip segment rows must be 3..256");
+    } else {
+        return fail(NLSE_EINVAL,
+                    "batch kernel kind must be 0 (LDS tiles), 1 (row-sweep tiles) or 2 (register strips)");
+    }
+    return NLSE_OK;
 }
 
 // (re)build the tile list and per-problem descriptors for the current kernel
@@ -369,13 +385,8 @@ int nlse_batch_sync(nlse_batch* b)
 int nlse_batch_set_kernel(nlse_batch* b, int kind, int rows)
 {
     if (!b) return fail(NLSE_EINVAL, "null batch");
-    if (kind == KERNEL_TILES) {
-        if (rows != 8 && rows != 16 && rows != 32) return fail(NLSE_EINVAL, "LDS tile rows must be 8, 16 or 32");
-    } else if (kind == KERNEL_ROWS) {
-        if (rows != 8 && rows != 12 && rows != 16) return fail(NLSE_EINVAL, "row-sweep tile rows must be 8, 12 or 16");
-    } else {
-        return fail(NLSE_EINVAL, "batch kernel kind must be 0 (LDS tiles) or 1 (row-sweep tiles)");
-    }
+    const int rc = check_rows(kind, rows);
+    if (rc != NLSE_OK) return rc;
     b->kern = kind;
     b->tile = rows;
     return build_tiles(b);
@@ -396,10 +407,8 @@ int nlse_batch_configure(nlse_batch* b, int stages_per_launch, int tile_rows, in
     if (!b) return fail(NLSE_EINVAL, "null batch");
     if (stages_per_launch != 1 && stages_per_launch != 2 && stages_per_launch != 4)
         return fail(NLSE_EINVAL, "stages per launch must be 1, 2 or 4");
-    if (b->kern == KERNEL_TILES && tile_rows != 8 && tile_rows != 16 && tile_rows != 32)
-        return fail(NLSE_EINVAL, "LDS tile rows must be 8, 16 or 32");
-    if (b->kern == KERNEL_ROWS && tile_rows != 8 && tile_rows != 12 && tile_rows != 16)
-        return fail(NLSE_EINVAL, "row-sweep tile rows must be 8, 12 or 16");
+    const int rc = check_rows(b->kern, tile_rows);
+    if (rc != NLSE_OK) return rc;
     if (grid < 8 || grid % 8 != 0) return fail(NLSE_EINVAL, "grid must be a positive multiple of 8");
     if (steps_per_graph < 0) return fail(NLSE_EINVAL, "steps per graph must be >= 0");
     b->grid = grid;
@@ -408,7 +417,7 @@ int nlse_batch_configure(nlse_batch* b, int stages_per_launch, int tile_rows, in
     if (tile_rows != b->tile || stages_per_launch != b->fuse) {
         b->fuse = stages_per_launch;
         b->tile = tile_rows;
-        return build_tiles(b);  // the row-sweep tile width depends on the stages per launch
+        return build_tiles(b);  // row-sweep tile and strip widths depend on the stages per launch
     }
     return NLSE_OK;
 }
@@ -428,11 +437,12 @@ int nlse_batch_autotune(nlse_batch* b, int steps, int* best_fusion, int* best_ti
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
     const int fz[3] = {1, 2, 4}, occ[3] = {2, 4, 8};
-    const int tl[2][3] = {{8, 16, 32}, {8, 12, 16}};  // rows per tile of each kernel family
+    // rows per tile (strips: per segment) of each kernel family
+    const int tl[3][3] = {{8, 16, 32}, {8, 12, 16}, {16, 32, 64}};
     const int gsave = b->gsteps;
     int bf = b->fuse, bk = b->kern, bt = b->tile, bg = b->grid, rc = NLSE_OK;
     double bms = 1e30;
-    for (int kind = 0; kind < 2 && rc == NLSE_OK; ++kind)
+    for (int kind = 0; kind < 3 && rc == NLSE_OK; ++kind)
         for (int x = 0; x < 3 && rc == NLSE_OK; ++x)
             for (int y = 0; y < 3 && rc == NLSE_OK; ++y)
                 for (int z = 0; z < 3 && rc == NLSE_OK; ++z) {

@@ -797,14 +797,15 @@ int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, d
     HIPCHK(hipMemcpyAsync(saved, h->U, bytes, hipMemcpyDeviceToDevice, h->stream));
     if (h->Ulo) HIPCHK(hipMemcpyAsync((char*)saved + bytes, h->Ulo, bytes, hipMemcpyDeviceToDevice, h->stream));
     // candidates: every kernel family and tile height for 1, 2 and 4 stages
-    // per launch (register strips for 1 and 2)
+    // per launch
     struct Cand { int fuse, kern, tile; };
     const Cand cand[] = {{1, KERNEL_TILES, 8},  {1, KERNEL_TILES, 16}, {1, KERNEL_TILES, 32}, {2, KERNEL_TILES, 8},
                          {2, KERNEL_TILES, 16}, {2, KERNEL_TILES, 32}, {4, KERNEL_TILES, 8},  {4, KERNEL_TILES, 16},
                          {4, KERNEL_TILES, 32}, {1, KERNEL_ROWS, 8},   {1, KERNEL_ROWS, 12},  {1, KERNEL_ROWS, 16},
                          {2, KERNEL_ROWS, 8},   {2, KERNEL_ROWS, 12},  {2, KERNEL_ROWS, 16},  {4, KERNEL_ROWS, 8},
                          {4, KERNEL_ROWS, 12},  {4, KERNEL_ROWS, 16},  {1, KERNEL_STRIPS, 10}, {1, KERNEL_STRIPS, 16},
-                         {2, KERNEL_STRIPS, 10}, {2, KERNEL_STRIPS, 16}};
+                         {2, KERNEL_STRIPS, 10}, {2, KERNEL_STRIPS, 16}, {4, KERNEL_STRIPS, 32},
+                         {4, KERNEL_STRIPS, 64}};
     const int gsave = h->gsteps;
     h->gsteps = 0;
     int bf = h->fuse, bk = h->kern, bt = h->tile, rc = NLSE_OK;

@@ -28,8 +28,7 @@ template <typename T> struct StageArgs {
 //   KERNEL_TILES   LDS tiles of 64 columns x 8/16/32 rows, generic 2D thread map
 //   KERNEL_ROWS    row-sweep LDS tiles, lane = column, 8/12/16 rows (nlse2d_rows.h)
 //   KERNEL_STRIPS  register-marching strips, one wave each, 3..256-row segments,
-//                  1 or 2 stages per launch (nlse2d_strip.h); 4 stages per launch
-//                  use KERNEL_ROWS
+//                  1, 2 or 4 stages per launch (nlse2d_strip.h)
 enum { KERNEL_TILES = 0, KERNEL_ROWS = 1, KERNEL_STRIPS = 2 };
 
 template <typename T> hipError_t launch_stage1d(const StageArgs<T>& a, hipStream_t st);
@@ -55,7 +54,8 @@ struct BatchProb2 {
 };
 
 // One persistent launch of RK4 stages first..last over every tile of a batch
-// of 2D problems (tile heights 8, 16 or 32; `grid` workgroups, a multiple of 8).
+// of 2D problems (`tile` = rows per tile, or per strip segment for
+// KERNEL_STRIPS; `grid` workgroups, a multiple of 8).
 template <typename T> struct BatchArgs {
     const c2<T>* yin;
     const c2<T>* u;
@@ -67,7 +67,7 @@ template <typename T> struct BatchArgs {
     int ntiles;
     c2<T>* ulo;
     int method, first, last, tile, grid;
-    int kernel;  // KERNEL_TILES or KERNEL_ROWS
+    int kernel;  // KERNEL_TILES, KERNEL_ROWS or KERNEL_STRIPS
 };
 template <typename T> hipError_t launch_batch2d(const BatchArgs<T>& a, hipStream_t st);
 

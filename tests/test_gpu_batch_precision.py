@@ -70,7 +70,8 @@ def _batch_inputs(seed, method):
 @pytest.mark.parametrize("kind,fusion,tile,grid,graph", [(0, 1, 8, 64, 0), (0, 2, 8, 1024, 0), (0, 2, 16, 256, 4),
                                                          (0, 4, 32, 8, 0), (0, 4, 16, 2048, 3), (1, 1, 8, 64, 0),
                                                          (1, 2, 8, 1024, 0), (1, 2, 12, 256, 4), (1, 4, 16, 8, 0),
-                                                         (1, 4, 8, 2048, 3)])
+                                                         (1, 4, 8, 2048, 3), (2, 1, 3, 64, 0), (2, 2, 16, 1024, 0),
+                                                         (2, 4, 32, 256, 4), (2, 4, 128, 8, 3), (2, 4, 5, 2048, 0)])
 def test_fp64_batch_is_bitwise_each_problem_alone(hip, oracle, method, kind, fusion, tile, grid, graph):
     _announce("batch64", method, kind, fusion, tile, grid, graph)
     probs = _batch_inputs(10 + method, method)
@@ -124,7 +125,8 @@ def test_compensated_single_precision_accuracy(hip, oracle, dim, shape, method):
     assert drift < 1e-7
 
 
-@pytest.mark.parametrize("kind,fusion,tile", [(0, 2, 8), (0, 4, 16), (0, 1, 32), (1, 2, 8), (1, 4, 12), (1, 1, 16)])
+@pytest.mark.parametrize("kind,fusion,tile", [(0, 2, 8), (0, 4, 16), (0, 1, 32), (1, 2, 8), (1, 4, 12), (1, 1, 16),
+                                              (2, 4, 64), (2, 2, 16), (2, 1, 5), (2, 4, 7)])
 def test_compensated_batch_equals_single_problem(hip, kind, fusion, tile):
     _announce("batch32c", kind, fusion, tile)
     probs = _batch_inputs(3, 2)
@@ -151,7 +153,8 @@ def test_batch_autotune_keeps_state(hip, oracle):
     for i, p in enumerate(probs):
         B.set_state(i, p["U"], p["V"])
     f, t, g, ms = B.autotune(2)
-    assert f in (1, 2, 4) and t in (8, 16, 32) and g % 8 == 0 and ms > 0
+    assert f in (1, 2, 4) and t in (8, 12, 16, 32, 64) and g % 8 == 0 and ms > 0
+    assert B.config()[0] == f and B.config()[2] == t
     B.step(3)
     for i, p in enumerate(probs):
         exp = oracle.take_steps(p["U"], p["V"], p["s"], p["a"], p["h2"], 1, 3, p["k"], 2)
@@ -166,7 +169,7 @@ def test_solver_autotune_keeps_state(hip, oracle):
     sol = hip.Solver((70, 90), 2, 2, 2, -1.0, 1.0, 0.01, 0.002)
     sol.set_state(U, V)
     f, t, ms = sol.autotune(2)
-    assert f in (1, 2, 4) and t in (8, 16, 32)
+    assert f in (1, 2, 4) and t in (8, 10, 12, 16, 32, 64)
     sol.step(4)
     assert bits_equal(sol.state(), oracle.take_steps(U, V, -1.0, 1.0, 0.01, 2, 4, 0.002, 2))
     sol.close()

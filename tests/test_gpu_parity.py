@@ -189,8 +189,8 @@ def _kernel_cases():
     """Row-sweep tiles (kind 1) and register strips (kind 2): grids whose sizes
     hit every decomposition edge (n mod rows == 1, M mod W == 1 for W = 48..62,
     a last strip or tile of two columns, one strip/segment/tile for the whole
-    grid), both methods, both precisions, every BC, 1/2/4 stages per launch
-    (strips: 1/2), every tile height and 3..32-row strip segments."""
+    grid), both methods, both precisions, every BC, 1/2/4 stages per launch,
+    every tile height and 3..32-row strip segments."""
     out = []
     seed = 9000
     shapes = [(3, 3), (4, 5), (3, 100), (100, 3), (9, 57), (17, 61), (33, 63), (5, 113), (7, 121),
@@ -203,7 +203,7 @@ def _kernel_cases():
                     seed += 1
                     out.append((shape, method, prec, BC, (1, 2, 4)[seed % 3], 1, (8, 12, 16)[(seed // 3) % 3], seed))
                     seed += 1
-                    out.append((shape, method, prec, BC, 1 + seed % 2, 2, strips[seed % len(strips)], seed))
+                    out.append((shape, method, prec, BC, (1, 2, 4)[seed % 3], 2, strips[seed % len(strips)], seed))
     return out
 
 
@@ -232,7 +232,8 @@ def test_kernel_families_2d_match_oracle(hip, oracle, shape, method, prec, BC, f
     assert bits_equal(got, exp), f"max |diff| {np.max(np.abs(got - exp))}"
 
 
-@pytest.mark.parametrize("kind,fusion,rows", [(1, 1, 8), (1, 2, 8), (1, 2, 16), (1, 4, 12), (2, 1, 16), (2, 2, 10)])
+@pytest.mark.parametrize("kind,fusion,rows", [(1, 1, 8), (1, 2, 8), (1, 2, 16), (1, 4, 12), (2, 1, 16), (2, 2, 10),
+                                              (2, 4, 64)])
 @pytest.mark.parametrize("method", [1, 2])
 def test_kernel_families_full_size_2d_bitwise(hip, oracle, method, kind, fusion, rows):
     """1024 x 1024 dark vortex, MSD, a few steps through the row-sweep tiles
