@@ -203,9 +203,10 @@ int nlse_vortex_profile(int m, double OM, double a, double s, double hr, int n, 
  * not available for 1D N <= 4096, which runs whole chunks in one kernel) */
 int nlse_profile_launches(nlse_solver *h, int steps, double *ms_launch, int *nlaunch);
 /* RK4 stages fused per kernel launch (the results are identical for every
- * choice): 2D 1, 2 or 4 (default 2); 3D 1 or 2 (default 2, used by CD with a
- * boundary other than MSD: two stages per plane-marching launch; 2SHOC and
- * MSD run one stage per launch) */
+ * choice): 2D 1, 2 or 4 (default 2); 3D 1 or 2 (two stages per
+ * plane-marching launch, CD and 2SHOC, with a boundary other than MSD; MSD
+ * runs one stage per launch; default 2, except 1 for 2SHOC in double
+ * precision, where the pair kernel is slower) */
 int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
 /* 2D: the stage kernel family and its rows per tile (results identical for
  * every choice):
@@ -214,9 +215,11 @@ int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
  *           8 (default), 12 or 16 rows
  *   kind 2  register-marching strips (one wave per strip, no LDS), segments
  *           of 3..256 rows, for 1, 2 or 4 stages per launch
- * 3D 2SHOC: kind 1 (default) marches planes with D kept on chip, tiles of 4
- * (default) or 8 rows x 60 columns; kind 0 runs a D pass and an F pass per
- * stage.  MSD boundaries always use the two passes.
+ * 3D 2SHOC, one stage per launch: kind 1 (default) marches planes with D
+ * kept on chip, tiles of 4 (default) or 8 rows x 60 columns; kind 0 runs a D
+ * pass and an F pass per stage.  MSD boundaries always use the two passes.
+ * Two stages per launch (nlse_set_fusion) march tiles of 4 rows x 56
+ * columns (2SHOC) or 4 x 60 (CD) whatever the kind.
  * 1D: kind 1 (default) advances `rows` steps (1..96 CD, 1..48 2SHOC; default
  * 32) per launch on 256-point windows with a halo, one per CU; kind 0 runs a
  * whole chunk in one workgroup (N <= 4096) or one launch per stage. */
@@ -229,8 +232,9 @@ int nlse_get_config(const nlse_solver *h, int *fusion, int *kind, int *rows);
 /* 2D: time `steps` steps for every configuration (stages per launch x kernel
  * kind x rows) on a saved copy of the state, keep the fastest, and restore
  * the state; reports the stages per launch, the rows and the ms per step
- * (any pointer may be NULL; nlse_get_config gives the kind).  1D/3D: no-op
- * that reports the stages per launch in use. */
+ * (any pointer may be NULL; nlse_get_config gives the kind).  3D (not MSD):
+ * the same between one and two stages per plane-marching launch.  1D and 3D
+ * MSD: no-op that reports the stages per launch in use. */
 int nlse_autotune(nlse_solver *h, int steps, int *best_fusion, int *best_tile, double *best_ms);
 /* replay the chunk loop from a captured hipGraph of `steps_per_graph` steps
  * (0 = plain stream launches, the default) */

@@ -443,7 +443,7 @@ class Solver:
         _check(self._lib.nlse_set_tile(self._h, int(tile_rows)))
 
     def set_fusion(self, stages_per_launch):
-        """RK4 stages per kernel launch: 2D 1, 2 or 4; 3D 1 or 2 (CD, not MSD)."""
+        """RK4 stages per kernel launch: 2D 1, 2 or 4; 3D 1 or 2 (not MSD)."""
         _check(self._lib.nlse_set_fusion(self._h, int(stages_per_launch)))
 
     def set_kernel(self, kind, rows):

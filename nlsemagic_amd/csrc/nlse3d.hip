@@ -16,6 +16,7 @@
 // coalesced line read.
 #include "nlse3d_cdpair.h"
 #include "nlse3d_march.h"
+#include "nlse3d_shocpair.h"
 #include "nlse_common.h"
 #include "nlse_launch.h"
 
@@ -177,9 +178,47 @@ __global__ __launch_bounds__(NW * 64) void cdpair3d_kernel(const c2<T>* __restri
     cdpair_tile<T, FIRST, TJ, NW>(sm, ci * g.IC, jt * TJ, kt * 60, g, yin, u, acc, yout, V, c, BC, ulo);
 }
 
+// 2SHOC, two stages per launch on plane-marching tiles (one extended row per wave)
+template <typename T, int FIRST, int TJ, int NW>
+__global__ __launch_bounds__(NW * 64) void shocpair3d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+                                                             c2<T>* __restrict__ acc, c2<T>* __restrict__ yout,
+                                                             const T* __restrict__ V, Coefs<T> c, March3 g, int BC,
+                                                             c2<T>* ulo)
+{
+    __shared__ ShocPairLds<T, TJ> sm;
+    const int t = (blockIdx.x & 7) * g.per_xcd + (blockIdx.x >> 3);
+    if (t >= g.tj * g.tk * g.ti) return;
+    const int kt = t % g.tk, rest = t / g.tk;
+    const int jt = rest % g.tj, ci = rest / g.tj;
+    shocpair_tile<T, FIRST, TJ, NW>(sm, ci * g.IC, jt * TJ, kt * 56, g, yin, u, acc, yout, V, c, BC, ulo);
+}
+
+template <typename T>
+static hipError_t launch_shocpair3d(const StageArgs<T>& a, bool first, hipStream_t st)
+{
+    constexpr int TJ = std::is_same<T, float>::value ? 8 : 4, NW = TJ + 8;
+    March3 g;
+    g.N = a.N;
+    g.M = a.M;
+    g.L = a.L;
+    g.IC = 32;
+    g.tj = (a.N + TJ - 1) / TJ;
+    g.tk = (a.M + 55) / 56;
+    g.ti = (a.L + g.IC - 1) / g.IC;
+    g.per_xcd = (g.tj * g.tk * g.ti + 7) / 8;
+    if (first)
+        hipLaunchKernelGGL((shocpair3d_kernel<T, 1, TJ, NW>), dim3(8 * g.per_xcd), dim3(NW * 64), 0, st, a.yin, a.u,
+                           a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+    else
+        hipLaunchKernelGGL((shocpair3d_kernel<T, 3, TJ, NW>), dim3(8 * g.per_xcd), dim3(NW * 64), 0, st, a.yin, a.u,
+                           a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_pair3d(const StageArgs<T>& a, bool first, hipStream_t st)
 {
+    if (a.method == 2) return launch_shocpair3d<T>(a, first, st);
     constexpr int TJ = 4, NW = 4;
     March3 g;
     g.N = a.N;

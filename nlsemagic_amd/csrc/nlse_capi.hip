@@ -95,12 +95,12 @@ void drop_graphs(nlse_solver* h)
     }
 }
 
-// RK4 stages per launch actually used: 2D 1, 2 or 4; 3D CD pairs of stages
+// RK4 stages per launch actually used: 2D 1, 2 or 4; 3D pairs of stages
 // (not MSD, whose partner may lie in another plane); 1 otherwise
 int fusion(const nlse_solver* h)
 {
     if (h->dim == 2) return h->fuse;
-    if (h->dim == 3 && h->method == 1 && h->BC != BC_MSD && h->fuse == 2) return 2;
+    if (h->dim == 3 && h->BC != BC_MSD && h->fuse == 2) return 2;
     return 1;
 }
 
@@ -455,7 +455,10 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
         return fail(NLSE_EHIP, "stream/event creation failed");
     }
     h->ubase = h->U;
-    h->fuse = (dim >= 2) ? 2 : 1;  // 3D: CD pairs of stages (used where the method and BC allow)
+    // 2D: pairs of stages; 3D: pairs (where the BC allows) except 2SHOC in
+    // double precision, whose pair kernel runs at half the one-stage rate
+    // (12 waves per workgroup, registers spilled)
+    h->fuse = (dim == 1 || (dim == 3 && method == 2 && precision == 2)) ? 1 : 2;
     h->kern = KERNEL_ROWS;         // the fastest family on a 1024^2 grid and for 3D 2SHOC
     h->tile = (dim == 3) ? 4 : (dim == 1 ? 32 : 8);  // 3D: plane-march rows; 1D: steps per launch
     set_params(h, BC, s, a, h2, k);
@@ -784,7 +787,7 @@ int nlse_get_config(const nlse_solver* h, int* fusion_out, int* kind, int* rows)
 int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, double* best_ms)
 {
     if (!h || steps <= 0) return fail(NLSE_EINVAL, "bad argument");
-    if (h->dim != 2) {  // 1D and 3D: nothing to tune; report what is in use
+    if (h->dim == 1 || (h->dim == 3 && h->BC == BC_MSD)) {  // nothing to tune; report what is in use
         if (best_fusion) *best_fusion = fusion(h);
         if (best_tile) *best_tile = 0;
         if (best_ms) *best_ms = 0;
@@ -796,10 +799,11 @@ int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, d
     if (hipMalloc(&saved, sbytes) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
     HIPCHK(hipMemcpyAsync(saved, h->U, bytes, hipMemcpyDeviceToDevice, h->stream));
     if (h->Ulo) HIPCHK(hipMemcpyAsync((char*)saved + bytes, h->Ulo, bytes, hipMemcpyDeviceToDevice, h->stream));
-    // candidates: every kernel family and tile height for 1, 2 and 4 stages
-    // per launch
+    // candidates: 2D every kernel family and tile height for 1, 2 and 4
+    // stages per launch; 3D one stage (plane marching) or two per launch
     struct Cand { int fuse, kern, tile; };
-    const Cand cand[] = {{1, KERNEL_TILES, 8},  {1, KERNEL_TILES, 16}, {1, KERNEL_TILES, 32}, {2, KERNEL_TILES, 8},
+    const Cand cand3[] = {{1, KERNEL_ROWS, 4}, {2, KERNEL_ROWS, 4}};
+    const Cand cand2[] = {{1, KERNEL_TILES, 8},  {1, KERNEL_TILES, 16}, {1, KERNEL_TILES, 32}, {2, KERNEL_TILES, 8},
                          {2, KERNEL_TILES, 16}, {2, KERNEL_TILES, 32}, {4, KERNEL_TILES, 8},  {4, KERNEL_TILES, 16},
                          {4, KERNEL_TILES, 32}, {1, KERNEL_ROWS, 8},   {1, KERNEL_ROWS, 12},  {1, KERNEL_ROWS, 16},
                          {2, KERNEL_ROWS, 8},   {2, KERNEL_ROWS, 12},  {2, KERNEL_ROWS, 16},  {4, KERNEL_ROWS, 8},
@@ -810,7 +814,10 @@ int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, d
     h->gsteps = 0;
     int bf = h->fuse, bk = h->kern, bt = h->tile, rc = NLSE_OK;
     double bms = 1e30;
-    for (const Cand& cd : cand) {
+    const Cand* cand = h->dim == 3 ? cand3 : cand2;
+    const int ncand = h->dim == 3 ? 2 : (int)(sizeof(cand2) / sizeof(cand2[0]));
+    for (int ci = 0; ci < ncand; ++ci) {
+        const Cand& cd = cand[ci];
         if (rc != NLSE_OK) break;
         h->fuse = cd.fuse;
         h->kern = cd.kern;

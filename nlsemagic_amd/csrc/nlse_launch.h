@@ -40,7 +40,7 @@ template <typename T> hipError_t launch_stage3d(const StageArgs<T>& a, hipStream
 // a whole step in one launch (yin == u; the new U goes to yout != u).
 template <typename T> hipError_t launch_pair2d(const StageArgs<T>& a, bool first, hipStream_t st);
 template <typename T> hipError_t launch_step2d(const StageArgs<T>& a, hipStream_t st);
-// 3D CD, stages 1-2 (first) or 3-4 in one plane-marching launch; not MSD
+// 3D, stages 1-2 (first) or 3-4 in one plane-marching launch (CD or 2SHOC); not MSD
 template <typename T> hipError_t launch_pair3d(const StageArgs<T>& a, bool first, hipStream_t st);
 
 // tiles of problem p are [tile0, tile0 + tiles_i*tiles_j); fields of every

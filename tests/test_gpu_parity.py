@@ -313,6 +313,7 @@ def test_march3d_matches_oracle_and_two_pass(hip, oracle, shape, prec, BC, rows,
     res = []
     for kind in (1, 0):
         sol = hip.Solver(shape, 2, prec, BC, s, 0.9, h2, k)
+        sol.set_fusion(1)
         sol.set_kernel(kind, rows if kind == 1 else 8)
         sol.set_state(U, V)
         sol.step(3)
@@ -337,6 +338,7 @@ def test_march3d_compensated_matches_two_pass(hip, shape, BC):
     res = []
     for kind in (1, 0):
         sol = hip.Solver(shape, 2, 3, BC, -1.0, 0.9, 0.04, 0.004)
+        sol.set_fusion(1)
         sol.set_kernel(kind, 4 if kind == 1 else 8)
         sol.set_state(U, V)
         sol.step(7)
@@ -390,36 +392,40 @@ def test_block1d_matches_oracle(hip, oracle, n, method, BC, S, steps, prec, seed
     assert bits_equal(res[0], res[1]) and bits_equal(res[0], res[2])
 
 
-def _cdpair3d_cases():
-    """3D CD two-stage plane-marching launches: several i chunks (32 planes),
-    4-row j tiles with a last tile of one to three rows, 60-column k tiles
-    with a last tile of one or two columns, one plane/row/column of interior
-    only; Dirichlet, Lap0 and the zero boundary."""
+def _pair3d_cases():
+    """3D two-stage plane-marching launches (CD: 4-row x 60-column tiles,
+    2SHOC: 4 x 56 with a halo of four): several i chunks (32 planes), a last
+    j tile of one to three rows, a last k tile of one or two columns, one
+    plane/row/column of interior only; Dirichlet, Lap0 and the zero
+    boundary."""
     out = []
     seed = 40000
-    for shape in [(70, 20, 130), (33, 9, 61), (40, 17, 121), (3, 30, 64), (65, 3, 3), (34, 11, 62), (5, 6, 7)]:
-        for prec in (2, 1):
-            for BC in (1, 3, 0):
-                seed += 1
-                out.append((shape, prec, BC, seed))
+    for shape in [(70, 20, 130), (33, 9, 61), (40, 17, 121), (3, 30, 64), (65, 3, 3), (34, 11, 62), (5, 6, 7),
+                  (36, 5, 57), (9, 13, 113), (4, 4, 58)]:
+        for method in (1, 2):
+            for prec in (2, 1):
+                for BC in (1, 3, 0):
+                    seed += 1
+                    out.append((shape, method, prec, BC, seed))
     return out
 
 
-@pytest.mark.parametrize("shape,prec,BC,seed", _cdpair3d_cases())
-def test_cdpair3d_matches_oracle_and_one_stage(hip, oracle, shape, prec, BC, seed):
-    """Two CD stages per launch in 3D (the first stage's output kept on chip)
-    give the reference's bits and the bits of one stage per launch."""
-    _announce("cdpair3d", shape, prec, BC, seed)
+@pytest.mark.parametrize("shape,method,prec,BC,seed", _pair3d_cases())
+def test_pair3d_matches_oracle_and_one_stage(hip, oracle, shape, method, prec, BC, seed):
+    """Two stages per launch in 3D (the first stage's output and, for 2SHOC,
+    both stages' D kept on chip) give the reference's bits and the bits of
+    one stage per launch."""
+    _announce("pair3d", shape, method, prec, BC, seed)
     rng = np.random.default_rng(seed)
     U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
     V = 0.5 * rng.uniform(-1, 1, shape)
     h2 = 0.04
     k = 0.1 * h2
     s = -1.0 if BC != 3 else 0.8
-    exp = oracle.take_steps(U, V, s, 0.9, h2, BC, 3, k, 1, precision=prec)
+    exp = oracle.take_steps(U, V, s, 0.9, h2, BC, 3, k, method, precision=prec)
     res = []
     for fusion in (2, 1):
-        sol = hip.Solver(shape, 1, prec, BC, s, 0.9, h2, k)
+        sol = hip.Solver(shape, method, prec, BC, s, 0.9, h2, k)
         sol.set_fusion(fusion)
         assert sol.config()[0] == fusion
         sol.set_state(U, V)
@@ -435,56 +441,60 @@ def test_cdpair3d_matches_oracle_and_one_stage(hip, oracle, shape, prec, BC, see
 
 @pytest.mark.parametrize("graph", [0, 3])
 @pytest.mark.parametrize("BC", [1, 0])
-def test_cdpair3d_resident_chunks_equal_one_call(hip, oracle, BC, graph):
+@pytest.mark.parametrize("method", [1, 2])
+def test_pair3d_resident_chunks_equal_one_call(hip, oracle, method, BC, graph):
     """State kept on the GPU across chunks of odd and even lengths, with and
     without hipGraph replay (an odd graph length is rounded up to even for the
     U/Y1 swap of a pair step), equals one long call."""
-    _announce("cdpair3d chunks", BC, graph)
-    rng = np.random.default_rng(41 + BC + graph)
+    _announce("pair3d chunks", method, BC, graph)
+    rng = np.random.default_rng(41 + BC + graph + 7 * method)
     shape = (36, 10, 70)
     U = 1 + 0.1 * rng.uniform(-1, 1, shape) + 0.1j * rng.uniform(-1, 1, shape)
     V = 0.1 * rng.uniform(-1, 1, shape)
     s, a, h2, k = -1.0, 1.0, 0.04, 0.004
-    sol = hip.Solver(shape, 1, 2, BC, s, a, h2, k)
+    sol = hip.Solver(shape, method, 2, BC, s, a, h2, k)
     sol.set_graph(graph)
+    sol.set_fusion(2)
     sol.set_state(U, V)
     assert sol.config()[0] == 2
     for n in (7, 1, 9, 4):
         sol.step(n)
     got = sol.state()
     sol.close()
-    exp = oracle.take_steps(U, V, s, a, h2, BC, 21, k, 1)
+    exp = oracle.take_steps(U, V, s, a, h2, BC, 21, k, method)
     assert bits_equal(got, exp)
 
 
-def test_cdpair3d_msd_runs_one_stage_per_launch(hip, oracle):
+@pytest.mark.parametrize("method", [1, 2])
+def test_pair3d_msd_runs_one_stage_per_launch(hip, oracle, method):
     """MSD in 3D pairs a boundary point with one in another plane, so a
     requested pair fusion runs one stage per launch (and stays bit-exact)."""
     shape = (9, 6, 65)
     rng = np.random.default_rng(3)
     U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
     V = 0.5 * rng.uniform(-1, 1, shape)
-    sol = hip.Solver(shape, 1, 2, 2, -1.0, 0.9, 0.04, 0.004)
+    sol = hip.Solver(shape, method, 2, 2, -1.0, 0.9, 0.04, 0.004)
     sol.set_fusion(2)
     assert sol.config()[0] == 1
     sol.set_state(U, V)
     sol.step(3)
     got = sol.state()
     sol.close()
-    assert bits_equal(got, oracle.take_steps(U, V, -1.0, 0.9, 0.04, 2, 3, 0.004, 1))
+    assert bits_equal(got, oracle.take_steps(U, V, -1.0, 0.9, 0.04, 2, 3, 0.004, method))
 
 
+@pytest.mark.parametrize("method", [1, 2])
 @pytest.mark.parametrize("shape,BC", [((70, 20, 130), 1), ((33, 9, 61), 3), ((40, 17, 121), 0)])
-def test_cdpair3d_compensated_matches_one_stage(hip, shape, BC):
-    """Compensated single precision (hi + lo state) through two CD stages per
+def test_pair3d_compensated_matches_one_stage(hip, shape, BC, method):
+    """Compensated single precision (hi + lo state) through two stages per
     launch equals one stage per launch bit for bit after several steps."""
-    _announce("cdpair3d comp", shape, BC)
+    _announce("pair3d comp", shape, BC, method)
     rng = np.random.default_rng(sum(shape) * 3 + BC)
     U = (0.9 + 0.3 * rng.uniform(-1, 1, shape)) + 1j * (0.3 * rng.uniform(-1, 1, shape))
     V = 0.5 * rng.uniform(-1, 1, shape)
     res = []
     for fusion in (2, 1):
-        sol = hip.Solver(shape, 1, 3, BC, -1.0, 0.9, 0.04, 0.004)
+        sol = hip.Solver(shape, method, 3, BC, -1.0, 0.9, 0.04, 0.004)
         sol.set_fusion(fusion)
         sol.set_state(U, V)
         sol.step(7)
@@ -493,7 +503,8 @@ def test_cdpair3d_compensated_matches_one_stage(hip, shape, BC):
     assert bits_equal(res[0], res[1])
 
 
-def test_cdpair3d_full_size_bitwise(hip, oracle):
+@pytest.mark.parametrize("method", [1, 2])
+def test_pair3d_full_size_bitwise(hip, oracle, method):
     """128^3 dark vortex ring, Lap0, two steps through two-stage launches:
     bit-exact at a size with many i chunks and tiles in every direction."""
     n, h = 128, 0.2
@@ -503,11 +514,34 @@ def test_cdpair3d_full_size_bitwise(hip, oracle):
     U = np.tanh(np.hypot(rho, Z)) * np.exp(1j * np.arctan2(Z, rho))
     V = np.zeros(U.shape)
     k = 0.05 * h * h
-    _announce("cdpair3d full", n)
-    sol = hip.Solver((n, n, n), 1, 2, 3, -1.0, 1.0, h * h, k)
+    _announce("pair3d full", n, method)
+    sol = hip.Solver((n, n, n), method, 2, 3, -1.0, 1.0, h * h, k)
     sol.set_state(U, V)
     sol.step(2)
     got = sol.state()
     sol.close()
-    exp = oracle.take_steps(U, V, -1.0, 1.0, h * h, 3, 2, k, 1)
+    exp = oracle.take_steps(U, V, -1.0, 1.0, h * h, 3, 2, k, method)
     assert bits_equal(got, exp), rel_l2(got, exp)
+
+
+@pytest.mark.parametrize("method,prec,default", [(1, 2, 2), (2, 2, 1), (2, 1, 2), (2, 3, 2)])
+def test_3d_default_fusion_and_autotune(hip, oracle, method, prec, default):
+    """3D runs two stages per launch by default, except 2SHOC in double
+    precision; the tuner picks between one and two on a saved copy of the
+    state and the run continues from the untouched state."""
+    shape = (40, 12, 70)
+    rng = np.random.default_rng(17 + method + prec)
+    U = 1 + 0.1 * rng.uniform(-1, 1, shape) + 0.1j * rng.uniform(-1, 1, shape)
+    V = 0.1 * rng.uniform(-1, 1, shape)
+    sol = hip.Solver(shape, method, prec, 1, -1.0, 1.0, 0.04, 0.004)
+    assert sol.config()[0] == default
+    sol.set_state(U, V)
+    f, t, ms = sol.autotune(2)
+    assert f in (1, 2) and ms > 0 and sol.config()[0] == f
+    sol.step(3)
+    got = sol.state()
+    sol.close()
+    if prec == 2:
+        assert bits_equal(got, oracle.take_steps(U, V, -1.0, 1.0, 0.04, 1, 3, 0.004, method))
+    else:
+        assert rel_l2(got, oracle.take_steps(U, V, -1.0, 1.0, 0.04, 1, 3, 0.004, method)) < 1e-6
