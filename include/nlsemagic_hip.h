@@ -226,8 +226,9 @@ int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
 int nlse_set_kernel(nlse_solver *h, int kind, int rows);
 /* nlse_set_kernel(h, 0, tile_rows) */
 int nlse_set_tile(nlse_solver *h, int tile_rows);
-/* the configuration in use: stages per launch, kernel kind, rows (any
- * pointer may be NULL) */
+/* the configuration in use: stages per launch, kernel kind and rows as set
+ * by nlse_set_kernel (2D rows per tile or strip segment, 3D march rows, 1D
+ * steps per launch); any pointer may be NULL */
 int nlse_get_config(const nlse_solver *h, int *fusion, int *kind, int *rows);
 /* 2D: time `steps` steps for every configuration (stages per launch x kernel
  * kind x rows) on a saved copy of the state, keep the fastest, and restore

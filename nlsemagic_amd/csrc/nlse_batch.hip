@@ -265,10 +265,14 @@ int nlse_batch_create(nlse_batch** out, int nprob, const int* N, const int* M, c
     }
     // cleared on the batch's own stream (a null-stream memset is not ordered
     // with it and could land after the caller's first uploads)
-    (void)hipMemsetAsync(b->U, 0, eb, b->stream);
-    (void)hipMemsetAsync(b->V, 0, eb / 2, b->stream);
-    if (b->Ulo) (void)hipMemsetAsync(b->Ulo, 0, eb, b->stream);
-    (void)hipStreamSynchronize(b->stream);
+    hipError_t e = hipMemsetAsync(b->U, 0, eb, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(b->V, 0, eb / 2, b->stream);
+    if (e == hipSuccess && b->Ulo) e = hipMemsetAsync(b->Ulo, 0, eb, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        nlse_batch_destroy(b);
+        return fail(NLSE_EHIP, std::string("initial clear: ") + hipGetErrorString(e));
+    }
     int rc = build_tiles(b);
     if (rc != NLSE_OK) {
         nlse_batch_destroy(b);

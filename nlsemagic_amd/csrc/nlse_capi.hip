@@ -59,9 +59,9 @@ struct nlse_solver {
     hipGraphExec_t gexec[2];  // per buffer holding U when the graph starts
     int gsteps;               // steps per graph replay (0: plain launches)
     int gcaptured[2];         // steps held by gexec[i]
-    int fuse;                 // 2D: RK4 stages per launch (1, 2 or 4)
-    int kern;                 // 2D: kernel family (KERNEL_TILES, KERNEL_ROWS, KERNEL_STRIPS)
-    int tile;                 // 2D: rows per tile / strip segment of that family
+    int fuse;                 // RK4 stages per launch requested (2D 1, 2 or 4; 3D 1 or 2)
+    int kern;                 // kernel family (KERNEL_TILES, KERNEL_ROWS, KERNEL_STRIPS)
+    int tile;                 // 2D rows per tile / strip segment; 3D march rows; 1D steps per launch
     void* ubase;              // the buffer U was allocated in (U swaps with Y1 when fuse > 1)
     hipEvent_t ev0, ev1;
 };
@@ -465,10 +465,14 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
     // a defined initial state: zero field and potential, cleared on the
     // solver's own stream (a null-stream memset is not ordered with it and
     // could land after the caller's first writes)
-    (void)hipMemsetAsync(h->U, 0, eb, h->stream);
-    (void)hipMemsetAsync(h->V, 0, eb / 2, h->stream);
-    if (h->Ulo) (void)hipMemsetAsync(h->Ulo, 0, eb, h->stream);
-    (void)hipStreamSynchronize(h->stream);
+    hipError_t e = hipMemsetAsync(h->U, 0, eb, h->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(h->V, 0, eb / 2, h->stream);
+    if (e == hipSuccess && h->Ulo) e = hipMemsetAsync(h->Ulo, 0, eb, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        nlse_destroy(h);
+        return fail(NLSE_EHIP, std::string("initial clear: ") + hipGetErrorString(e));
+    }
     *out = h;
     return NLSE_OK;
 }
@@ -779,8 +783,8 @@ int nlse_get_config(const nlse_solver* h, int* fusion_out, int* kind, int* rows)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");
     if (fusion_out) *fusion_out = fusion(h);
-    if (kind) *kind = h->dim == 2 ? h->kern : 0;
-    if (rows) *rows = h->dim == 2 ? h->tile : 0;
+    if (kind) *kind = h->kern;
+    if (rows) *rows = h->tile;
     return NLSE_OK;
 }
 
@@ -802,7 +806,7 @@ int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, d
     // candidates: 2D every kernel family and tile height for 1, 2 and 4
     // stages per launch; 3D one stage (plane marching) or two per launch
     struct Cand { int fuse, kern, tile; };
-    const Cand cand3[] = {{1, KERNEL_ROWS, 4}, {2, KERNEL_ROWS, 4}};
+    const Cand cand3[] = {{1, KERNEL_ROWS, 4}, {1, KERNEL_ROWS, 8}, {2, KERNEL_ROWS, 4}};
     const Cand cand2[] = {{1, KERNEL_TILES, 8},  {1, KERNEL_TILES, 16}, {1, KERNEL_TILES, 32}, {2, KERNEL_TILES, 8},
                          {2, KERNEL_TILES, 16}, {2, KERNEL_TILES, 32}, {4, KERNEL_TILES, 8},  {4, KERNEL_TILES, 16},
                          {4, KERNEL_TILES, 32}, {1, KERNEL_ROWS, 8},   {1, KERNEL_ROWS, 12},  {1, KERNEL_ROWS, 16},
@@ -815,7 +819,7 @@ int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, d
     int bf = h->fuse, bk = h->kern, bt = h->tile, rc = NLSE_OK;
     double bms = 1e30;
     const Cand* cand = h->dim == 3 ? cand3 : cand2;
-    const int ncand = h->dim == 3 ? 2 : (int)(sizeof(cand2) / sizeof(cand2[0]));
+    const int ncand = h->dim == 3 ? (int)(sizeof(cand3) / sizeof(cand3[0])) : (int)(sizeof(cand2) / sizeof(cand2[0]));
     for (int ci = 0; ci < ncand; ++ci) {
         const Cand& cd = cand[ci];
         if (rc != NLSE_OK) break;

@@ -545,3 +545,15 @@ def test_3d_default_fusion_and_autotune(hip, oracle, method, prec, default):
         assert bits_equal(got, oracle.take_steps(U, V, -1.0, 1.0, 0.04, 1, 3, 0.004, method))
     else:
         assert rel_l2(got, oracle.take_steps(U, V, -1.0, 1.0, 0.04, 1, 3, 0.004, method)) < 1e-6
+
+
+def test_config_reports_kernel_choice_in_every_dimension(hip):
+    """The kernel kind and rows set by set_kernel are what config() reports,
+    in 1D (steps per launch), 2D (rows per tile or strip segment) and 3D
+    (plane-march rows)."""
+    for shape, kind, rows in (((300,), 1, 17), ((300,), 0, 8), ((20, 30), 2, 12), ((20, 30), 1, 16),
+                              ((10, 12, 70), 1, 8), ((10, 12, 70), 0, 8)):
+        sol = hip.Solver(shape, 2, 2, 1, -1.0, 1.0, 0.04, 0.004)
+        sol.set_kernel(kind, rows)
+        assert sol.config()[1:] == (kind, rows), shape
+        sol.close()
