@@ -224,6 +224,38 @@ def run_batched(args):
     print(json.dumps(line))
 
 
+def dropin_host_rate(nm, U, V, k, calls=5):
+    """The drop-in boundary as the reference's callers use it: one
+    NLSE2D_TAKE_STEPS_HIP call per chunk of 50 steps on host buffers (device
+    allocation, upload over PCIe, the steps, download, free), timed on the
+    host over `calls` calls after one warm-up call.  Not the headline value,
+    which is measured with the state resident in HBM."""
+    import ctypes as C
+    L = nm.lib()
+    dp = C.POINTER(C.c_double)
+    ur = np.ascontiguousarray(U.real)
+    ui = np.ascontiguousarray(U.imag)
+    v = np.ascontiguousarray(V)
+    outr = np.empty(U.shape)
+    outi = np.empty(U.shape)
+    args = (ur.ctypes.data_as(dp), ui.ctypes.data_as(dp), v.ctypes.data_as(dp), outr.ctypes.data_as(dp),
+            outi.ctypes.data_as(dp), -1.0, 1.0, H * H, 2, CHUNK, k, N, M, 2)
+
+    def call():
+        rc = L.NLSE2D_TAKE_STEPS_HIP(*args)
+        if rc != 0:
+            raise RuntimeError(f"NLSE2D_TAKE_STEPS_HIP failed: {L.nlse_last_error().decode()}")
+
+    call()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        call()
+    ms = (time.perf_counter() - t0) * 1e3 / calls
+    return {"what": f"one NLSE2D_TAKE_STEPS_HIP call per chunk of {CHUNK} steps on host buffers "
+                    "(allocate, upload, steps, download, free), mean of 5 calls",
+            "ms_per_call": round(ms, 3), "rate": N * M * CHUNK / (ms * 1e-3)}
+
+
 def run_2d_matrix(args):
     """The 2D configuration matrix on the headline grid (1024^2, MSD, the
     headline's initial state): {CD, 2SHOC} x {FP64, FP32, compensated FP32}
@@ -666,6 +698,8 @@ def main():
                 "launch_us_event_per_kernel": [round(x * 1e3, 2) for x in sol.profile_launches(20)]}
     sol.close()
 
+    dropin = dropin_host_rate(nm, U, V, k) if args.precision == 2 else None
+
     cpu = parity = None
     if not args.no_cpu and args.precision == 2:
         cpu, W_cpu, cpu_steps = cpu_baseline(U, V, k)
@@ -688,7 +722,7 @@ def main():
                    "grid": [N, M], "h": H, "k_over_h2": round(K_RULE, 6), "chunk_size": CHUNK,
                    "rk4_steps_timed": total_steps, "device_ms": dev_ms, "hip_graph_steps": args.graph,
                    "stages_per_launch": args.fusion, "kernel_kind": args.kernel, "tile_rows": args.tile,
-                   "mass_drift": mass_drift, "finite": finite},
+                   "mass_drift": mass_drift, "finite": finite, "dropin_host_buffers": dropin},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
