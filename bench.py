@@ -548,6 +548,24 @@ def run_grid(args):
     print(json.dumps(line))
 
 
+def run_3d_counters(args):
+    """A short run for hardware-counter passes (rocprofv3 --pmc): --steps RK4
+    steps of one 3D variant (--method, --precision, --fusion, --tile) on the
+    256^3 workload's state, with no timing and no CPU work."""
+    import nlsemagic_amd as nm
+    U, h = vortex_rings_3d()
+    k = (0.8 if args.method == 1 else 0.8 * 0.75) * h * h / (3 * np.sqrt(2))
+    sol = nm.Solver(U.shape, args.method, args.precision, 1, -1.0, 1.0, h * h, k)
+    sol.set_fusion(args.fusion)
+    sol.set_kernel(args.kernel, args.tile or 4)
+    sol.set_state(U, np.zeros(U.shape))
+    sol.step(args.steps)
+    sol.sync()
+    print(json.dumps({"workload": "3d-counters", "method": args.method, "precision": args.precision,
+                      "config": sol.config(), "steps": args.steps}))
+    sol.close()
+
+
 def run_serving(args):
     """Poisson request stream (seed 2) through the continuous-batching server,
     then the same requests one at a time through the plain FP64 path."""
@@ -598,6 +616,7 @@ def main():
                     help="RK4 stages per kernel launch (results are identical)")
     ap.add_argument("--precision", type=int, default=2, choices=(1, 2, 3),
                     help="2d workload: 2 FP64 (the headline), 1 FP32, 3 compensated FP32")
+    ap.add_argument("--method", type=int, default=2, choices=(1, 2), help="3d-counters: 1 CD, 2 2SHOC")
     ap.add_argument("--kernel", type=int, default=1, choices=(0, 1, 2),
                     help="2D stage kernel: 0 LDS tiles, 1 row-sweep tiles, 2 register strips")
     ap.add_argument("--tile", type=int, default=None,
@@ -609,7 +628,7 @@ def main():
     ap.add_argument("--qps", type=float, default=4.0, help="serving: Poisson arrival rate")
     ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
     ap.add_argument("--workload", default="2d",
-                    choices=("2d", "2d-matrix", "batched", "1d", "3d", "serving", "vorticity"),
+                    choices=("2d", "2d-matrix", "batched", "1d", "3d", "3d-counters", "serving", "vorticity"),
                     help="2d: the 1024^2 FP64 2SHOC headline; batched: 512 uneven problems, FP32c vs FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -629,6 +648,9 @@ def main():
         return
     if args.workload in ("1d", "3d"):
         run_grid(args)
+        return
+    if args.workload == "3d-counters":
+        run_3d_counters(args)
         return
     if args.workload == "2d-matrix":
         run_2d_matrix(args)
