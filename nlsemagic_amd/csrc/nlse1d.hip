@@ -266,8 +266,12 @@ __global__ __launch_bounds__(1024) void block1d_kernel(const c2<T>* __restrict__
                                                        const T* __restrict__ Vg, Coefs<T> c, int N, int BC,
                                                        int steps, int halo)
 {
-    __shared__ c2<T> sY[1024 + 2];
-    __shared__ c2<T> sD[(METHOD == 2) ? 1024 + 2 : 1];
+    // stage input and D double-buffered by stage parity: a buffer is
+    // rewritten two stages later, after every thread has passed the barrier
+    // of the stage in between, so one barrier per stage (two for 2SHOC)
+    // separates each write from its readers
+    __shared__ c2<T> sY[2][1024 + 2];
+    __shared__ c2<T> sD[(METHOD == 2) ? 2 : 1][(METHOD == 2) ? 1024 + 2 : 1];
     const int o0 = blockIdx.x * BLK_SEG, o1 = min(o0 + BLK_SEG, N);
     const int w0 = max(o0 - halo, 0), w1 = min(o1 + halo, N);
     const int t = threadIdx.x;
@@ -276,35 +280,39 @@ __global__ __launch_bounds__(1024) void block1d_kernel(const c2<T>* __restrict__
     const int ic = act ? i : w1 - 1;
     c2<T> U = uin[ic], Lo = COMP ? loin[ic] : mk<T>(0, 0), A = mk<T>(0, 0), Y = U;
     const T Vr = Vg[ic];
-    // window-local accessors: one padding element each side (stale values
-    // beyond the window only reach points within the halo)
-    auto Yl = [&](int gi) { return sY[gi - w0 + 1]; };
     auto Vl = [&](int gi) { return gi == i ? Vr : Vg[gi]; };
-    auto Dl = [&](int gi) { return sD[gi - w0 + 1]; };
-    auto rhs = make_rhs1d<T, METHOD>(Yl, Vl, Dl, c, N, BC);
-    if (t == 0) {
-        sY[0] = mk<T>(0, 0);
-        if (METHOD == 2) sD[0] = mk<T>(0, 0);
-    }
-    if (t == w1 - w0) {
-        sY[t + 1] = mk<T>(0, 0);
-        if (METHOD == 2) sD[t + 1] = mk<T>(0, 0);
+    // one padding element each side of the window (stale values beyond the
+    // window only reach points within the halo)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        if (t == 0) {
+            sY[b][0] = mk<T>(0, 0);
+            if (METHOD == 2) sD[b][0] = mk<T>(0, 0);
+        }
+        if (t == w1 - w0) {
+            sY[b][t + 1] = mk<T>(0, 0);
+            if (METHOD == 2) sD[b][t + 1] = mk<T>(0, 0);
+        }
     }
     for (int step = 0; step < steps; ++step) {
-#pragma unroll 1
-        for (int stage = 1; stage <= 4; ++stage) {
-            if (act) sY[t + 1] = Y;
+        static_for<0, 4>([&](auto qc) {
+            constexpr int stage = decltype(qc)::value + 1;
+            constexpr int b = (stage - 1) & 1;
+            auto Yl = [&](int gi) { return sY[b][gi - w0 + 1]; };
+            auto Dl = [&](int gi) { return sD[METHOD == 2 ? b : 0][gi - w0 + 1]; };
+            auto rhs = make_rhs1d<T, METHOD>(Yl, Vl, Dl, c, N, BC);
+            if (act) sY[b][t + 1] = Y;
             __syncthreads();
-            if (METHOD == 2) {
-                if (act) sD[t + 1] = rhs.d_at(i);
+            if constexpr (METHOD == 2) {
+                if (act) sD[b][t + 1] = rhs.d_at(i);
                 __syncthreads();
             }
             if (act) {
                 const c2<T> f = rhs.f_at(i);
-                if (stage == 1) {
+                if constexpr (stage == 1) {
                     A = f;
                     Y = mk<T>(U.x + c.k2 * f.x, U.y + c.k2 * f.y);
-                } else if (stage == 4) {
+                } else if constexpr (stage == 4) {
                     A.x = A.x + (T)1.0 * f.x;
                     A.y = A.y + (T)1.0 * f.y;
                     if constexpr (COMP) {
@@ -321,8 +329,7 @@ __global__ __launch_bounds__(1024) void block1d_kernel(const c2<T>* __restrict__
                     Y = mk<T>(U.x + kk * f.x, U.y + kk * f.y);
                 }
             }
-            __syncthreads();
-        }
+        });
     }
     if (act && i >= o0 && i < o1) {
         uout[i] = U;

@@ -30,14 +30,6 @@
 
 namespace nlse {
 
-template <int B, int E, class F> __device__ __forceinline__ void static_for(F&& f)
-{
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
 // DPP lane shifts over the whole 64-lane wave (GFX9 wavefront shifts)
 constexpr int DPP_NEXT = 0x130;  // wave_shl:1 -- lane l reads lane l+1 (column j+1)
 constexpr int DPP_PREV = 0x138;  // wave_shr:1 -- lane l reads lane l-1 (column j-1)

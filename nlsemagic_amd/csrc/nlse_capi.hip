@@ -460,7 +460,8 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
     // (12 waves per workgroup, registers spilled)
     h->fuse = (dim == 1 || (dim == 3 && method == 2 && precision == 2)) ? 1 : 2;
     h->kern = KERNEL_ROWS;         // the fastest family on a 1024^2 grid and for 3D 2SHOC
-    h->tile = (dim == 3) ? 4 : (dim == 1 ? 32 : 8);  // 3D: plane-march rows; 1D: steps per launch
+    // 3D: plane-march rows; 1D: steps per launch (the fastest on N = 2048)
+    h->tile = (dim == 3) ? 4 : (dim == 1 ? (method == 2 ? 16 : 32) : 8);
     set_params(h, BC, s, a, h2, k);
     // a defined initial state: zero field and potential, cleared on the
     // solver's own stream (a null-stream memset is not ordered with it and

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace nlse {
 
 // complex value stored interleaved (real, imaginary) in HBM and LDS
@@ -19,8 +21,18 @@ template <typename T> using c2 = typename C2<T>::type;
 
 template <typename T> __host__ __device__ inline c2<T> mk(T x, T y) { c2<T> r; r.x = x; r.y = y; return r; }
 
+// f(integral_constant<int, B>), ..., f(integral_constant<int, E-1>): a loop
+// whose index is a compile-time constant in every copy of the body
+template <int B, int E, class F> __device__ __forceinline__ void static_for(F&& f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // Scalars of one integrator call, precomputed on the host exactly as the
-// reference file of each dimension does (see coefs_* in nlse_capi.cpp).
+// reference file of each dimension does (coefs1d, coefs2d, coefs3d in nlse_host.h).
 template <typename T> struct Coefs {
     T s, a, k, k2, k6;
     T ah2;            // a/h^2 (CD)
