@@ -287,7 +287,9 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
             epilogue(k, e, ff);
             ROWS_SCHED_FENCE();
         }
-        __syncthreads();  // every read of this stage's Y and D rows is done
+        // every read of this stage's Y and D rows is done (after the last
+        // stage only the MSD boundary rows below still read LDS)
+        if (q < NF - 1 || top || bot) __syncthreads();
         if (top || bot) {
 #pragma unroll
             for (int k = 0; k < RW; ++k) {
