@@ -226,10 +226,11 @@ def run_batched(args):
 
 def dropin_host_rate(nm, U, V, k, calls=5):
     """The drop-in boundary as the reference's callers use it: one
-    NLSE2D_TAKE_STEPS_HIP call per chunk of 50 steps on host buffers (device
-    allocation, upload over PCIe, the steps, download, free), timed on the
-    host over `calls` calls after one warm-up call.  Not the headline value,
-    which is measured with the state resident in HBM."""
+    NLSE2D_TAKE_STEPS_HIP call per chunk of 50 steps on host buffers (upload
+    over PCIe into the device buffers the library keeps from the previous
+    call of this shape, the steps, download), timed on the host over `calls`
+    calls after one warm-up call.  Not the headline value, which is measured
+    with the state resident in HBM."""
     import ctypes as C
     L = nm.lib()
     dp = C.POINTER(C.c_double)
@@ -252,7 +253,7 @@ def dropin_host_rate(nm, U, V, k, calls=5):
         call()
     ms = (time.perf_counter() - t0) * 1e3 / calls
     return {"what": f"one NLSE2D_TAKE_STEPS_HIP call per chunk of {CHUNK} steps on host buffers "
-                    "(allocate, upload, steps, download, free), mean of 5 calls",
+                    f"(upload, steps, download; device buffers kept between calls), mean of {calls} calls",
             "ms_per_call": round(ms, 3), "rate": N * M * CHUNK / (ms * 1e-3)}
 
 
