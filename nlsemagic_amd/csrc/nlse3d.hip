@@ -219,7 +219,10 @@ template <typename T>
 hipError_t launch_pair3d(const StageArgs<T>& a, bool first, hipStream_t st)
 {
     if (a.method == 2) return launch_shocpair3d<T>(a, first, st);
-    constexpr int TJ = 4, NW = 4;
+    // FP64: one extended row per wave (stages 3-4 then fit in 117 VGPRs
+    // instead of 208, four waves per SIMD instead of two); single precision
+    // is faster with two rows per wave
+    constexpr int TJ = 4, NW = std::is_same<T, double>::value ? 8 : 4;
     March3 g;
     g.N = a.N;
     g.M = a.M;
