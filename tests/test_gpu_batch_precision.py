@@ -219,3 +219,23 @@ def test_blowup_is_counted(hip):
     mass, mxr, bad = sol.diagnostics()
     assert bad > 0
     sol.close()
+
+
+@pytest.mark.parametrize("BC", [2, 3, 0])
+@pytest.mark.parametrize("kind,fusion,tile", [(0, 2, 16), (1, 2, 8), (1, 4, 12), (2, 2, 16), (2, 4, 32), (2, 1, 3)])
+def test_fp64_batch_other_boundaries_bitwise(hip, oracle, BC, kind, fusion, tile):
+    """MSD, Lap0 and the zero boundary through every batch kernel family:
+    each problem bit-identical to the CPU oracle run alone."""
+    _announce("batch64 BC", BC, kind, fusion, tile)
+    probs = _batch_inputs(20 + BC, 2)
+    B = hip.Batch([p["shape"] for p in probs], [p["s"] for p in probs], [p["a"] for p in probs],
+                  [p["h2"] for p in probs], [p["k"] for p in probs], method=2, precision=2, BC=BC)
+    B.set_kernel(kind, tile)
+    B.configure(fusion, tile, 512, 0)
+    for i, p in enumerate(probs):
+        B.set_state(i, p["U"], p["V"])
+    B.step(4)
+    for i, p in enumerate(probs):
+        exp = oracle.take_steps(p["U"], p["V"], p["s"], p["a"], p["h2"], BC, 4, p["k"], 2)
+        assert bits_equal(B.state(i), exp), (i, p["shape"])
+    B.close()
