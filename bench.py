@@ -5,7 +5,7 @@ Workload (the reference's 2D headline case): 1024 x 1024 points on
 50 RK4 steps, initial state tanh(r) e^{i theta} + 1e-3 complex uniform noise
 (seed 0).  The reference states k = 0.3 h^2 for this case, which is beyond
 RK4's stability limit for 2SHOC (0.265 h^2/a; the solution overflows within
-~100 steps); the timed run uses the reference driver's own
+~100 steps, which --workload 2d-matrix reports); the timed run uses the reference driver's own
 rule k = 0.8 * 0.75 * h^2 / (2 sqrt 2) = 0.2121 h^2 (NLSEmagic2D.m:131-136).
 
 One bench "step" = one chunk of 50 RK4 steps on the device-resident state

@@ -26,8 +26,11 @@
  * gateway checks its arguments and reports errors with mexErrMsgTxt.
  *
  * Uses the separate-complex MEX API (mxGetPi), which Octave's mkoctfile
- * --mex and MATLAB's default (-R2017b) API provide.  Build recipes:
- * INTEGRATION.md.
+ * --mex and MATLAB's default (-R2017b) API provide.  One build per MATLAB
+ * name, e.g. NLSE2D_TAKE_STEPS_2SHOC_CUDA_D:
+ *   mkoctfile --mex -DNLSE_DIM=2 -DNLSE_METHOD=2 -DNLSE_PREC=2 -Iinclude \
+ *     nlse_mex_gateway.c -Lnlsemagic_amd -lnlsemagic_hip -o NLSE2D_TAKE_STEPS_2SHOC_CUDA_D.mex
+ * and NLSEmagic2D_Take_Steps_GPU with -DNLSE_DIM=2 -DNLSE_V2.
  */
 #include <stdlib.h>
 #include <string.h>
