@@ -178,7 +178,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 c2<T> dd = z;
                 if (interior(g)) {
                     const c2<T> a = sy(e + 1, 0), b = sy(e - 1, 0);
-                    const c2<T> yp = lane_shift2<DPP_NEXT, T>(yx), ym = lane_shift2<DPP_PREV, T>(yx);
+                    const c2<T> yp = sy(e, 1), ym = sy(e, -1);  // the row's k +- 1 neighbours from its LDS copy
                     dd = mk<T>((a.x + b.x - 4 * yx.x + yp.x + ym.x) * c.lh2, (a.y + b.y - 4 * yx.y + yp.y + ym.y) * c.lh2);
                     if (colbc) {
                         c2<T> db = z;
@@ -258,7 +258,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 const T n = nlin<T>(c.s, yx, vx);
                 if constexpr (METHOD == 2) {
                     const c2<T> dc = sd(e, 0), da = sd(e + 1, 0), db = sd(e - 1, 0);
-                    const c2<T> dl = lane_shift2<DPP_NEXT, T>(dc), dr = lane_shift2<DPP_PREV, T>(dc);
+                    const c2<T> dl = sd(e, 1), dr = sd(e, -1);
                     const c2<T> aa = sy(e + 1, 1), ab = sy(e + 1, -1), ba = sy(e - 1, 1), bb = sy(e - 1, -1);
                     ff = mk<T>(-c.a * dc.y + c.a1_12 * (da.y + db.y + dl.y + dr.y)
                                    - c.a1_6h2 * (aa.y + ab.y - 4 * yx.y + ba.y + bb.y) - n * yx.y,
@@ -266,7 +266,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                                    + c.a1_6h2 * (aa.x + ab.x - 4 * yx.x + ba.x + bb.x) + n * yx.x);
                 } else {
                     const c2<T> a = sy(e + 1, 0), b = sy(e - 1, 0);
-                    const c2<T> l = lane_shift2<DPP_NEXT, T>(yx), rt = lane_shift2<DPP_PREV, T>(yx);
+                    const c2<T> l = sy(e, 1), rt = sy(e, -1);
                     ff = mk<T>(-c.ah2 * (a.y - 4 * yx.y + b.y + l.y + rt.y) - n * yx.y,
                                c.ah2 * (a.x - 4 * yx.x + b.x + l.x + rt.x) + n * yx.x);
                 }
