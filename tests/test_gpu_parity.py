@@ -503,10 +503,12 @@ def test_pair3d_compensated_matches_one_stage(hip, shape, BC, method):
     assert bits_equal(res[0], res[1])
 
 
+@pytest.mark.parametrize("prec", [2, 1])
 @pytest.mark.parametrize("method", [1, 2])
-def test_pair3d_full_size_bitwise(hip, oracle, method):
-    """128^3 dark vortex ring, Lap0, two steps through two-stage launches:
-    bit-exact at a size with many i chunks and tiles in every direction."""
+def test_pair3d_full_size_bitwise(hip, oracle, method, prec):
+    """128^3 dark vortex ring, Lap0, two steps through two-stage launches
+    (both precisions; single precision runs 2SHOC on 8-row tiles): bit-exact
+    at a size with many i chunks and tiles in every direction."""
     n, h = 128, 0.2
     x = -12.8 + h * np.arange(n)
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
@@ -514,13 +516,17 @@ def test_pair3d_full_size_bitwise(hip, oracle, method):
     U = np.tanh(np.hypot(rho, Z)) * np.exp(1j * np.arctan2(Z, rho))
     V = np.zeros(U.shape)
     k = 0.05 * h * h
-    _announce("pair3d full", n, method)
-    sol = hip.Solver((n, n, n), method, 2, 3, -1.0, 1.0, h * h, k)
+    _announce("pair3d full", n, method, prec)
+    sol = hip.Solver((n, n, n), method, prec, 3, -1.0, 1.0, h * h, k)
+    sol.set_fusion(2)
+    assert sol.config()[0] == 2
     sol.set_state(U, V)
     sol.step(2)
     got = sol.state()
     sol.close()
-    exp = oracle.take_steps(U, V, -1.0, 1.0, h * h, 3, 2, k, method)
+    exp = oracle.take_steps(U, V, -1.0, 1.0, h * h, 3, 2, k, method, precision=prec)
+    if prec == 1:
+        got = got.astype(np.complex64).astype(np.complex128)
     assert bits_equal(got, exp), rel_l2(got, exp)
 
 
