@@ -118,6 +118,21 @@ int nlse_query(nlse_solver *h);
  * out3[0] = sum |U|^2 (mass / h^d), out3[1] = max Re U, out3[2] = number of
  * non-finite points (the reference drivers' blow-up test) */
 int nlse_diagnostics(nlse_solver *h, double *out3);
+/* The reference drivers' full RK4 stability bound of the current state when
+ * they pick k themselves (k = 0): NLSEmagic1D_FRS.m:233-275, NLSE2D.m:427-477,
+ * NLSE3D.m:443-495.  With x = (h^2/a)(s|U|^2 - V) at every point and the
+ * scheme's eigenvalue offsets G,
+ *   kfull = (h^2/a) sqrt(8) / max(|Bmax|, max over G and points |x - G|),
+ * Bmax = 0 (Dirichlet), the real part of the largest-magnitude
+ * (h^2/(i a)) F/U on the first interior layer (MSD; F = the RHS, computed by
+ * one stage-1 launch into the solver's stage buffers), or max x on the
+ * boundary (Lap0).  out3 = {kfull, klin, k} with klin the linear bound and
+ * k = 0.9 min(kfull, klin) in 1D and 2D, 0.6 min(kfull, klin) in 3D. */
+int nlse_kfull(nlse_solver *h, double *out3);
+/* change the time step k of a solver (e.g. to nlse_kfull's k); the other
+ * parameters stay; no counterpart in the reference, whose drivers pass k on
+ * every chunk call */
+int nlse_set_step(nlse_solver *h, double k);
 /* 3D vorticity of the resident state (replaces the rmc_vorticity3D MEX file,
  * src/NLSEmagic3D/rmc_vorticity3D.c:24-354, as called by NLSE3D.m:594):
  * wx, wy, wz = curl of the velocity Im(conj(U) grad U) / (2|U|^2 + veps) and

@@ -123,6 +123,10 @@ def lib():
     L.nlse_batch_get_config.argtypes = [C.c_void_p, _I, _I, _I, _I]
     L.nlse_batch_autotune.restype = C.c_int
     L.nlse_batch_autotune.argtypes = [C.c_void_p, C.c_int, _I, _I, _I, C.POINTER(C.c_double)]
+    L.nlse_kfull.restype = C.c_int
+    L.nlse_kfull.argtypes = [C.c_void_p, _D]
+    L.nlse_set_step.restype = C.c_int
+    L.nlse_set_step.argtypes = [C.c_void_p, C.c_double]
     L.nlse_diagnostics.restype = C.c_int
     L.nlse_diagnostics.argtypes = [C.c_void_p, _D]
     L.nlse_batch_diagnostics.restype = C.c_int
@@ -354,6 +358,19 @@ class Solver:
         out = np.empty(3)
         _check(self._lib.nlse_diagnostics(self._h, _p(out, C.c_double)))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def kfull(self):
+        """(kfull, klin, k): the reference drivers' full RK4 stability bound of
+        the resident state, the linear bound, and the step they would take
+        (NLSE2D.m:427-477; see nlse_kfull).  MSD evaluates the RHS once into
+        the solver's stage buffers, which hold nothing between steps."""
+        out = np.empty(3)
+        _check(self._lib.nlse_kfull(self._h, _p(out, C.c_double)))
+        return float(out[0]), float(out[1]), float(out[2])
+
+    def set_step(self, k):
+        """Change the time step k (e.g. to the k of :meth:`kfull`)."""
+        _check(self._lib.nlse_set_step(self._h, float(k)))
 
     def vorticity(self, lh2, veps):
         """(wx, wy, wz, W) of the resident 3D state, as :func:`rmc_vorticity3D`

@@ -3,6 +3,8 @@
 // NLSE{1,2,3}D_TAKE_STEPS_HIP[_F] drop-in entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -592,6 +594,82 @@ int nlse_diagnostics(nlse_solver* h, double* out3)
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(NLSE_EHIP, std::string("diagnostics: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
+
+int nlse_set_step(nlse_solver* h, double k)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    if (!(k > 0) || !std::isfinite(k)) return fail(NLSE_EINVAL, "time step k must be positive and finite");
+    HIPCHK(hipStreamSynchronize(h->stream));  // steps already enqueued keep their k
+    set_params(h, h->BC, h->s, h->a, h->h2, k);
+    return NLSE_OK;
+}
+
+int nlse_kfull(nlse_solver* h, double* out3)
+{
+    if (!h || !out3) return fail(NLSE_EINVAL, "null argument");
+    // eigenvalue offsets G of the scheme's stability analysis, per dimension
+    // and method (NLSEmagic1D_FRS.m:236-242, NLSE2D.m:430-436, NLSE3D.m:447-453)
+    static const double g1cd[] = {4, 3, 1, 0};
+    static const double g1sh[] = {64, 63, 46, 12, -3, -4};
+    static const double g2cd[] = {8, 7, 6, 2, 1, 0};
+    static const double g2sh[] = {128, 127, 126, 110, 109, 92, 24, 9, 8, -6, -7, -8};
+    static const double g3cd[] = {12, 11, 10, 9, 3, 2, 1, 0};
+    static const double g3sh[] = {192, 191, 190, 189, 174, 173, 172, 156, 155, 138, 36, 21, 20, 6, 5, 4, -9, -10, -11, -12};
+    const double* G;
+    int ng;
+    double gs;  // the 2SHOC tables are given in twelfths
+    if (h->method == 1) {
+        G = h->dim == 1 ? g1cd : (h->dim == 2 ? g2cd : g3cd);
+        ng = h->dim == 1 ? 4 : (h->dim == 2 ? 6 : 8);
+        gs = 1.0;
+    } else {
+        G = h->dim == 1 ? g1sh : (h->dim == 2 ? g2sh : g3sh);
+        ng = h->dim == 1 ? 6 : (h->dim == 2 ? 12 : 20);
+        gs = 1.0 / 12;
+    }
+    // MSD needs F on the first interior layer: stage 1 leaves it in ktot
+    // (it also overwrites the stage buffer Y0; neither holds data between steps)
+    if (h->BC == BC_MSD) {
+        const int rc = enqueue_stage(h, 1);
+        if (rc != NLSE_OK) return rc;
+    }
+    KfGrid g;
+    g.n = h->n;
+    g.dim = h->dim;
+    g.ext[0] = h->dim == 3 ? h->L : h->N;
+    g.ext[1] = h->dim == 3 ? h->N : h->M;
+    g.ext[2] = h->M;
+    g.c = h->h2 / h->a;
+    g.s = h->s;
+    g.BC = h->BC;
+    double* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, sizeof(double) * (6 * 1024 + 6)));
+    hipError_t e = launch_kfull(h->U, h->Ulo, h->V, h->BC == BC_MSD ? h->acc : nullptr, is_double(h), g, d + 6, d,
+                                h->stream);
+    double r[6];
+    if (e == hipSuccess) e = hipMemcpyAsync(r, d, sizeof(r), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("kfull: ") + hipGetErrorString(e));
+    // Lmax = max over G of max over points |x - G|; Bmax by the BC
+    double lmax = 0;
+    for (int i = 0; i < ng; ++i) {
+        const double gv = G[i] * gs;
+        lmax = std::max(lmax, std::max(std::fabs(r[0] - gv), std::fabs(r[1] - gv)));
+    }
+    double bmax = 0;
+    if (h->BC == BC_MSD) bmax = r[3] >= 0 ? r[5] : 0.0;
+    else if (h->BC == BC_LAP0) bmax = r[2];
+    const double kfull = h->h2 / h->a * std::sqrt(8.0) / std::max(std::fabs(bmax), lmax);
+    // the linear bound and the step the drivers take (0.9 of the smaller in
+    // 1D and 2D, 0.6 in 3D)
+    const double klin = (h->method == 1 ? 1.0 : 0.75) * h->h2 / (h->dim * std::sqrt(2.0) * h->a);
+    const double f = h->dim == 3 ? 0.6 : 0.9;
+    out3[0] = kfull;
+    out3[1] = klin;
+    out3[2] = (kfull < klin) ? f * kfull : f * klin;
     return NLSE_OK;
 }
 

@@ -96,4 +96,153 @@ hipError_t launch_diag(const void* u, const void* ulo, bool dbl, long n, double*
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// The reduction behind the reference drivers' full RK4 stability bound
+// (NLSEmagic1D_FRS.m:244-263, NLSE2D.m:447-470, NLSE3D.m:456-479).  Per point
+// x = (h^2/a)(s|U|^2 - V); out = {min x, max x, max x over the boundary
+// points, and for the first interior layer the value z = (h^2/(i a)) F/U of
+// largest magnitude, ties to the larger angle as MATLAB's max orders complex
+// numbers: |z|, angle z, Re z}.  NaN values are skipped, as MATLAB's max
+// omits them.  F is the stage-1 RHS of the same state (ktot after stage 1).
+
+constexpr int KF_VALS = 6;
+
+struct KfAcc {
+    double xmin, xmax, bmax, zmag, zang, zre;
+    __device__ void init()
+    {
+        xmin = INFINITY;
+        xmax = -INFINITY;
+        bmax = -INFINITY;
+        zmag = -1.0;
+        zang = 0.0;
+        zre = 0.0;
+    }
+    __device__ void merge(const KfAcc& o)
+    {
+        xmin = fmin(xmin, o.xmin);
+        xmax = fmax(xmax, o.xmax);
+        bmax = fmax(bmax, o.bmax);
+        if (o.zmag > zmag || (o.zmag == zmag && o.zang > zang)) {
+            zmag = o.zmag;
+            zang = o.zang;
+            zre = o.zre;
+        }
+    }
+    __device__ void load(const double* v)
+    {
+        xmin = v[0]; xmax = v[1]; bmax = v[2]; zmag = v[3]; zang = v[4]; zre = v[5];
+    }
+    __device__ void store(double* v) const
+    {
+        v[0] = xmin; v[1] = xmax; v[2] = bmax; v[3] = zmag; v[4] = zang; v[5] = zre;
+    }
+};
+
+// one block's 256 accumulators folded in a fixed order (deterministic)
+__device__ __forceinline__ void kf_block_reduce(KfAcc a, double* out)
+{
+    __shared__ double sv[KF_VALS][256];
+    auto put = [&](int t, const KfAcc& x) {
+        sv[0][t] = x.xmin; sv[1][t] = x.xmax; sv[2][t] = x.bmax;
+        sv[3][t] = x.zmag; sv[4][t] = x.zang; sv[5][t] = x.zre;
+    };
+    auto get = [&](int t) {
+        KfAcc x;
+        x.xmin = sv[0][t]; x.xmax = sv[1][t]; x.bmax = sv[2][t];
+        x.zmag = sv[3][t]; x.zang = sv[4][t]; x.zre = sv[5][t];
+        return x;
+    };
+    put(threadIdx.x, a);
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            KfAcc x = get(threadIdx.x);
+            x.merge(get(threadIdx.x + w));
+            put(threadIdx.x, x);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) get(0).store(out);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kfull_partial_kernel(const c2<T>* __restrict__ u, const c2<T>* __restrict__ ulo,
+                                                            const T* __restrict__ V, const c2<T>* __restrict__ F,
+                                                            KfGrid g, double* __restrict__ part)
+{
+    KfAcc a;
+    a.init();
+    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < g.n; p += (long)gridDim.x * 256) {
+        const c2<T> v = u[p];
+        double re = v.x, im = v.y;
+        if (ulo) {
+            const c2<T> l = ulo[p];
+            re += (double)l.x;
+            im += (double)l.y;
+        }
+        // MATLAB: (hmin^2/a) * (s*U.*conj(U) - V)
+        const double x = g.c * (g.s * (re * re + im * im) - (double)V[p]);
+        if (!isnan(x)) {
+            a.xmin = fmin(a.xmin, x);
+            a.xmax = fmax(a.xmax, x);
+        }
+        // coordinates (slowest first) and their extents
+        int ci[3], cn[3];
+        long r = p;
+        for (int d = g.dim - 1; d >= 0; --d) {
+            cn[d] = g.ext[d];
+            ci[d] = (int)(r % g.ext[d]);
+            r /= g.ext[d];
+        }
+        bool bnd = false, inner = true, near = false;
+        for (int d = 0; d < g.dim; ++d) {
+            bnd = bnd || ci[d] == 0 || ci[d] == cn[d] - 1;
+            inner = inner && ci[d] >= 1 && ci[d] <= cn[d] - 2;
+            near = near || ci[d] == 1 || ci[d] == cn[d] - 2;
+        }
+        if (g.BC == BC_LAP0 && bnd && !isnan(x)) a.bmax = fmax(a.bmax, x);
+        if (g.BC == BC_MSD && F && inner && near) {
+            // z = (c/i) F/U: complex division F/U, then multiplication by -i c
+            const c2<T> f = F[p];
+            const double fr = f.x, fi = f.y, den = re * re + im * im;
+            const double qr = (fr * re + fi * im) / den, qi = (fi * re - fr * im) / den;
+            const double zr = g.c * qi, zi = -g.c * qr;
+            const double mag = hypot(zr, zi), ang = atan2(zi, zr);
+            if (!isnan(mag) && (mag > a.zmag || (mag == a.zmag && ang > a.zang))) {
+                a.zmag = mag;
+                a.zang = ang;
+                a.zre = zr;
+            }
+        }
+    }
+    kf_block_reduce(a, part + (long)KF_VALS * blockIdx.x);
+}
+
+__global__ __launch_bounds__(256) void kfull_fold_kernel(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    KfAcc a;
+    a.init();
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        KfAcc o;
+        o.load(part + (long)KF_VALS * b);
+        a.merge(o);
+    }
+    kf_block_reduce(a, out);
+}
+
+hipError_t launch_kfull(const void* u, const void* ulo, const void* V, const void* F, bool dbl, const KfGrid& g,
+                        double* part, double* out, hipStream_t st)
+{
+    const int nb = (int)std::min<long>(DIAG_BLOCKS, (g.n + 255) / 256);
+    if (dbl)
+        hipLaunchKernelGGL((kfull_partial_kernel<double>), dim3(nb), dim3(256), 0, st, (const double2*)u,
+                           (const double2*)nullptr, (const double*)V, (const double2*)F, g, part);
+    else
+        hipLaunchKernelGGL((kfull_partial_kernel<float>), dim3(nb), dim3(256), 0, st, (const float2*)u,
+                           (const float2*)ulo, (const float*)V, (const float2*)F, g, part);
+    hipLaunchKernelGGL(kfull_fold_kernel, dim3(1), dim3(256), 0, st, part, nb, out);
+    return hipGetLastError();
+}
+
 }  // namespace nlse

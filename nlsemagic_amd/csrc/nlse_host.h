@@ -127,6 +127,19 @@ int download_planes(const c2<T>* src, T* Ur, T* Ui, T* stage, long n, hipStream_
 // diagnostics of n points (u, plus ulo for a compensated state) into out[0..2]
 // on the device: sum |U|^2, max Re U, non-finite count; part: 3*1024 doubles
 hipError_t launch_diag(const void* u, const void* ulo, bool dbl, long n, double* part, double* out, hipStream_t st);
+
+// The kfull reduction (nlse_diag.hip): grid of `dim` axes of extents ext[]
+// (slowest first, n points), c = h^2/a, the nonlinearity s and the BC.
+struct KfGrid {
+    long n;
+    int dim, ext[3];
+    double c, s;
+    int BC;
+};
+// enqueue it into out[0..5] (device); `part` holds 6 * DIAG_BLOCKS doubles;
+// F (the stage-1 RHS of the state, same precision) is read for MSD only
+hipError_t launch_kfull(const void* u, const void* ulo, const void* V, const void* F, bool dbl, const KfGrid& g,
+                        double* part, double* out, hipStream_t st);
 // 3D vorticity of a state (L x N x M, k fastest) in double precision into
 // wx, wy, wz, W (device pointers, any may be null); ulo: low part of a
 // compensated single-precision state (or null).  No scratch memory.
