@@ -219,10 +219,12 @@ template <typename T>
 hipError_t launch_pair3d(const StageArgs<T>& a, bool first, hipStream_t st)
 {
     if (a.method == 2) return launch_shocpair3d<T>(a, first, st);
-    // FP64: one extended row per wave (stages 3-4 then fit in 117 VGPRs
-    // instead of 208, four waves per SIMD instead of two); single precision
-    // is faster with two rows per wave
-    constexpr int TJ = 4, NW = std::is_same<T, double>::value ? 8 : 4;
+    // 12 owned rows on 16 waves, one extended row per wave: few registers
+    // per wave (stages 3-4 in FP64: 117 VGPRs, against 208 with two rows per
+    // wave) and j-halo rows re-read through L2 for 12 owned rows instead of
+    // 4 (the halo re-reads dominate the HBM traffic beyond the 128 B per
+    // point-step the pair needs)
+    constexpr int TJ = 12, NW = TJ + 4;
     March3 g;
     g.N = a.N;
     g.M = a.M;
