@@ -16,7 +16,12 @@
 //     loads its segment plus a halo of 4*R*S points (R = 1 CD, 2 2SHOC: the
 //     reach of one stage), runs the S steps on chip like the resident kernel,
 //     and writes back its own points; values near the window edge go stale
-//     by R points per stage and never reach the owned points.  Launches
+//     by R points per stage and never reach the owned points.  The boundary
+//     formulas reach up to three points inward in one stage (MSD through the
+//     partner's stencil, the one-sided BC 4 difference), further than R, so
+//     the segment origins shift back until the last segment owns at least
+//     four points: a boundary point's inward reach then lands on owned
+//     points, whose dependence on the window stays R per stage.  Launches
 //     alternate between two state buffers.
 // Arithmetic follows the reference operand order with FP contraction off.
 #include <algorithm>
@@ -264,7 +269,7 @@ template <typename T, int METHOD, bool COMP>
 __global__ __launch_bounds__(1024) void block1d_kernel(const c2<T>* __restrict__ uin, c2<T>* __restrict__ uout,
                                                        const c2<T>* __restrict__ loin, c2<T>* __restrict__ loout,
                                                        const T* __restrict__ Vg, Coefs<T> c, int N, int BC,
-                                                       int steps, int halo)
+                                                       int steps, int halo, int soff)
 {
     // stage input and D double-buffered by stage parity: a buffer is
     // rewritten two stages later, after every thread has passed the barrier
@@ -272,7 +277,7 @@ __global__ __launch_bounds__(1024) void block1d_kernel(const c2<T>* __restrict__
     // separates each write from its readers
     __shared__ c2<T> sY[2][1024 + 2];
     __shared__ c2<T> sD[(METHOD == 2) ? 2 : 1][(METHOD == 2) ? 1024 + 2 : 1];
-    const int o0 = blockIdx.x * BLK_SEG, o1 = min(o0 + BLK_SEG, N);
+    const int o0 = max((int)blockIdx.x * BLK_SEG - soff, 0), o1 = min((int)(blockIdx.x + 1) * BLK_SEG - soff, N);
     const int w0 = max(o0 - halo, 0), w1 = min(o1 + halo, N);
     const int t = threadIdx.x;
     const int i = w0 + t;
@@ -346,23 +351,27 @@ hipError_t launch_block1d(const c2<T>* uin, c2<T>* uout, const c2<T>* loin, c2<T
     const int R = (method == 2) ? 2 : 1;
     const int halo = 4 * R * halo_steps;
     const int nb = (N + BLK_SEG - 1) / BLK_SEG;
+    // segment b owns [b*BLK_SEG - soff, (b+1)*BLK_SEG - soff) clipped to [0, N):
+    // the last segment owns at least 4 points (see block1d_kernel)
+    const int tail = N % BLK_SEG;
+    const int soff = (nb > 1 && tail > 0 && tail < 4) ? 4 - tail : 0;
     const int win = std::min(N, BLK_SEG + 2 * halo);
     const dim3 blk(((win + 63) / 64) * 64);
     const bool comp = loin != nullptr;
     if (method == 2) {
         if (comp)
             hipLaunchKernelGGL((block1d_kernel<T, 2, true>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
-                               BC, steps, halo);
+                               BC, steps, halo, soff);
         else
             hipLaunchKernelGGL((block1d_kernel<T, 2, false>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
-                               BC, steps, halo);
+                               BC, steps, halo, soff);
     } else {
         if (comp)
             hipLaunchKernelGGL((block1d_kernel<T, 1, true>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
-                               BC, steps, halo);
+                               BC, steps, halo, soff);
         else
             hipLaunchKernelGGL((block1d_kernel<T, 1, false>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
-                               BC, steps, halo);
+                               BC, steps, halo, soff);
     }
     return hipGetLastError();
 }

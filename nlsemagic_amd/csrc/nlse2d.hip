@@ -101,9 +101,8 @@ __global__ __launch_bounds__(NW * 64) void rows2d_kernel(const c2<T>* __restrict
 {
     __shared__ RowsLds<T, METHOD, FIRST, LAST, TI> sm;
     constexpr int W = RowsCfg<METHOD, FIRST, LAST, TI>::W;
-    const int t = g.map.tile();
-    if (t >= g.map.ntiles) return;
-    const int ti = t / g.tiles_j, tj = t - ti * g.tiles_j;
+    int ti, tj;
+    if (!g.edge_first(blockIdx.x, ti, tj)) return;
     const int i0 = ti * TI - g.ioff, j0 = tj * W - g.joff;
     if (rows_inner<METHOD, FIRST, LAST, TI>(i0, j0, g.N, g.M))
         rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true, NW>(sm, i0, j0, g.N, g.M, yin, u, acc, out, V, c, BC, ulo);
@@ -125,16 +124,17 @@ static hipError_t launch_rows_ti(const StageArgs<T>& a, hipStream_t st)
     g.tiles_j = (a.M + g.joff + W - 1) / W;
     g.map.ntiles = g.tiles_i * g.tiles_j;
     g.map.per_xcd = (g.map.ntiles + 7) / 8;
+    const int nwg = g.edge_first_setup();
     constexpr int NW = rows_waves<METHOD, FIRST, LAST, TI>();
     if constexpr (std::is_same<T, float>::value && LAST == 4) {
         if (a.ulo) {
-            hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, true, NW>), dim3(8 * g.map.per_xcd),
-                               dim3(NW * 64), 0, st, a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+            hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, true, NW>), dim3(nwg), dim3(NW * 64), 0,
+                               st, a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, false, NW>), dim3(8 * g.map.per_xcd), dim3(NW * 64),
-                       0, st, a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
+    hipLaunchKernelGGL((rows2d_kernel<T, METHOD, FIRST, LAST, TI, false, NW>), dim3(nwg), dim3(NW * 64), 0, st,
+                       a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
     return hipGetLastError();
 }
 

@@ -18,8 +18,13 @@
 //     to LDS, barrier; the last stage writes its rows to HBM.
 // MSD boundary rows take their partner row's D/F from LDS after the barrier
 // that completes it (D: one extra barrier, only in tiles holding row 0 or
-// N-1); MSD boundary columns take the partner lane's values by DPP.  Every
-// owned point gets the reference's operations in its operand order.
+// N-1).  MSD boundary columns take the partner lane's F by DPP; their D is
+// evaluated once per D phase rather than once per row: after the phase, one
+// wave holds (row, side) pairs in its lanes and computes every boundary
+// column point's D from the partner column's values in LDS (in-lane, each
+// row would spend a full wave of instructions, a division among them, on two
+// useful lanes).  Every owned point gets the reference's operations in its
+// operand order.
 #pragma once
 
 #include "nlse2d_strip.h"
@@ -73,6 +78,9 @@ struct RowsLds {
     // and TE-1 of y, which no F phase reads
     c2<T> y[Cfg::TE * 64 + 2];
     c2<T> d[Cfg::ND * 64 + 2];
+    // tiles holding column 0 or M-1 with MSD: V at columns 0, 1, M-2, M-1 of
+    // every extended row, for the D column pass
+    T bv[Cfg::TE][4];
 };
 
 // Tile with owned rows [i0, i0 + TI) and columns [j0, j0 + W) of an N x M
@@ -110,6 +118,9 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
     const int eg0 = INNER ? 0 : -g0, eg1 = INNER ? TE : N - g0;
     const bool msd = (BC == BC_MSD);
     const bool top = !INNER && msd && eg0 >= 0 && eg0 < TE, bot = !INNER && msd && eg1 - 1 >= 0 && eg1 - 1 < TE;
+    // MSD columns 0 / M-1 in this tile (uniform over the workgroup): their D
+    // comes from the column pass after each D phase
+    const bool colmsd = !INNER && msd && colbc;
     // a D/F row at grid row g is interior (the reference's full stencil)
     auto interior = [&](int g) { return INNER || (g > 0 && g < N - 1); };
     const c2<T> z = mk<T>(0, 0);
@@ -153,8 +164,21 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
         const int e = NW * k + w;
         if constexpr (FIRST == 1) Uc[k] = Yc[k];
         if (e < TE) sm.y[1 + e * 64 + lane] = Yc[k];
+        if (colmsd && e < TE) {
+            if (lane == lL) sm.bv[e][0] = Vc[k];
+            if (lane == lL + 1) sm.bv[e][1] = Vc[k];
+            if (lane == lR - 1) sm.bv[e][2] = Vc[k];
+            if (lane == lR) sm.bv[e][3] = Vc[k];
+        }
     }
     __syncthreads();
+    // the D column pass: lane l of wave 0 takes extended row l & 31 on side
+    // l >> 5 (0: column 0 at lane lL, partner column 1; 1: column M-1 at lane
+    // lR, partner column M-2) of the interior grid rows (rows 0 and N-1,
+    // corners included, belong to the row pass)
+    const int cp_e = lane & 31, cp_side = lane >> 5;
+    const int cp_lb = cp_side ? lR : lL, cp_lq = cp_side ? lR - 1 : lL + 1;
+    const bool cp_on = w == 0 && cp_lb >= 0 && cp_lb < 64 && cp_e < TE && interior(g0 + cp_e);
 
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
@@ -180,10 +204,9 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                     const c2<T> a = sy(e + 1, 0), b = sy(e - 1, 0);
                     const c2<T> yp = sy(e, 1), ym = sy(e, -1);  // the row's k +- 1 neighbours from its LDS copy
                     dd = mk<T>((a.x + b.x - 4 * yx.x + yp.x + ym.x) * c.lh2, (a.y + b.y - 4 * yx.y + yp.y + ym.y) * c.lh2);
-                    if (colbc) {
+                    if (colbc && !msd) {  // MSD columns: the column pass below
                         c2<T> db = z;
                         if (BC == BC_DIRICHLET) db = dir_d<T>(yx, vx, c);
-                        else if (msd) db = msd_d<T>(colq(dd), colq(yx), colq1(vx), yx, vx, c);
                         if (colL || colR) dd = db;
                     }
                 } else if (BC == BC_DIRICHLET) {
@@ -193,6 +216,15 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 ROWS_SCHED_FENCE();
             }
             __syncthreads();
+            if (colmsd) {  // MSD columns 0 / M-1 of the interior rows from columns 1 / M-2
+                if (cp_on && cp_e >= dlo && cp_e < dhi) {
+                    const int e = cp_e;
+                    sm.d[1 + (e - 1) * 64 + cp_lb] =
+                        msd_d<T>(sd(e, cp_lq - lane), sy(e, cp_lq - lane), sm.bv[e][cp_side ? 2 : 1], sy(e, cp_lb - lane),
+                                 sm.bv[e][cp_side ? 3 : 0], c);
+                }
+                __syncthreads();
+            }
             if (top || bot) {  // MSD rows 0 / N-1 from rows 1 / N-2 (uniform over the workgroup)
 #pragma unroll
                 for (int k = 0; k < RW; ++k) {

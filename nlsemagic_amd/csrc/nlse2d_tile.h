@@ -13,6 +13,51 @@ struct Grid2 {
     int tiles_i, tiles_j;
     int ioff, joff;  // tile origins shift by one when n mod tile == 1
     TileMap map;
+    // edge-first order (row-sweep tiles): the tiles of the first and last
+    // tile row and column (which hold the boundary conditions and take
+    // about twice as long) are dealt out first, so that the launch does not
+    // end on them; then the interior tiles, each XCD a contiguous block
+    int nb, nbp;          // edge tiles; nb rounded up to a multiple of 8
+    int ninner, per_xcd;  // interior tiles; per XCD
+    __device__ __forceinline__ bool edge_first(int b, int& ti, int& tj) const
+    {
+        if (b < nbp) {
+            if (b >= nb) return false;
+            if (ninner == 0) {
+                ti = b / tiles_j;
+                tj = b - ti * tiles_j;
+            } else if (b < 2 * tiles_j) {
+                ti = b < tiles_j ? 0 : tiles_i - 1;
+                tj = b < tiles_j ? b : b - tiles_j;
+            } else {
+                const int r = b - 2 * tiles_j;
+                ti = 1 + (r >> 1);
+                tj = (r & 1) ? tiles_j - 1 : 0;
+            }
+            return true;
+        }
+        const int bb = b - nbp, l = bb >> 3;
+        const int u = (bb & 7) * per_xcd + l;  // block b runs on XCD b mod 8 (nbp is a multiple of 8)
+        if (l >= per_xcd || u >= ninner) return false;
+        const int wi = tiles_j - 2, r = u / wi;
+        ti = 1 + r;
+        tj = 1 + (u - r * wi);
+        return true;
+    }
+    // workgroups of an edge-first launch
+    int edge_first_setup()
+    {
+        if (tiles_i <= 2 || tiles_j <= 2) {
+            nb = tiles_i * tiles_j;
+            ninner = 0;
+        } else {
+            nb = 2 * tiles_j + 2 * (tiles_i - 2);
+            ninner = (tiles_i - 2) * (tiles_j - 2);
+        }
+        nbp = (nb + 7) / 8 * 8;
+        per_xcd = (ninner + 7) / 8;
+        return nbp + 8 * per_xcd;
+    }
 };
 
 constexpr int TJ2 = 64;

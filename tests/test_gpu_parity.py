@@ -360,6 +360,14 @@ def _block1d_cases():
                 S = (1, 3, 8, 17, 48 if method == 2 else 96)[seed % 5]
                 steps = (1, 5, 37, 100)[seed % 4]
                 out.append((n, method, BC, S, steps, (2, 1, 3)[seed % 3], seed))
+    # the last 256-point segment would own one to three points: the MSD and
+    # one-sided boundary formulas reach further inward than a stage's R, and
+    # with one or two steps per launch that reach meets the window's stale edge
+    for n, BC in ((257, 2), (258, 4), (257, 4), (259, 4), (513, 2)):
+        for method in (1, 2):
+            for S in (1, 2):
+                seed += 1
+                out.append((n, method, BC, S, 4 * S, 2, seed))
     return out
 
 
