@@ -82,6 +82,20 @@ def initial_state():
     return U, np.zeros((N, M))
 
 
+def host_cpu():
+    """CPU model and thread counts of the host the CPU baseline runs on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(U, V, k, budget_s=12.0):
     """Reference CPU integrator, 1 thread, chunks of 10 steps until budget_s."""
     from oracle import oracle as O
@@ -98,7 +112,7 @@ def cpu_baseline(U, V, k, budget_s=12.0):
         dt = time.perf_counter() - t0
     finally:
         os.sched_setaffinity(0, prev)
-    return {"value": N * M * steps / dt, "unit": "grid-point-steps/s", "cores": 1,
+    return {"value": N * M * steps / dt, "unit": "grid-point-steps/s", "cores": 1, **host_cpu(),
             "kind": "reference" if use_ref else "port",
             "sample": f"{steps} RK4 steps of the same 1024x1024 2SHOC FP64 problem ({dt:.1f} s, 1 thread)"}, W, steps
 
@@ -447,7 +461,7 @@ def run_vorticity(args):
                      "kernel": "vort_march_kernel<double,16,16>", "launch_us": round(ms * 1e3, 2),
                      "algorithmic_bytes_per_point": alg, "design_bytes_per_point": need,
                      "design_frac": round(need * npts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "cpu_baseline": {"value": npts / cpu_s, "unit": "vorticity points/s", "cores": 1, "kind": "port",
+        "cpu_baseline": {"value": npts / cpu_s, "unit": "vorticity points/s", "cores": 1, **host_cpu(), "kind": "port",
                          "sample": f"the whole 256^3 state once ({cpu_s:.2f} s, oracle C restatement)"},
     }
     print(json.dumps(line))
@@ -527,6 +541,7 @@ def run_grid(args):
                    "mass_drift_fp64": {('CD' if m == 1 else '2SHOC'): fp64[m]["mass"] / m0 - 1 for m in methods},
                    "nonfinite": int(sum(v["bad"] for v in out.values())),
                    "cpu_rate_1thread": {('CD' if m == 1 else '2SHOC'): cpu[m] for m in methods},
+                   "cpu_host": host_cpu(),
                    "cpu_kind": "reference" if use_ref else "port"},
     }
     if args.workload == "3d":

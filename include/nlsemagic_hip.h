@@ -26,9 +26,10 @@
  *     (the reference returns void and checks nothing);
  *   - the double-precision results are bit-identical to the reference CPU
  *     integrators (NLSEmagic{1,2}D_CPU.c, NLSE3D_TAKE_STEPS_{CD,2SHOC}.c).
- * Device buffers are cached between calls of the same shape, so a caller
- * looping over chunks (as the reference MATLAB drivers do) does not pay an
- * allocation per call.  Calls are synchronous and thread-safe (serialised).
+ * Device buffers are cached between calls (the four most recently used
+ * shapes), so a caller looping over chunks (as the reference MATLAB drivers
+ * do), or alternating between a few problems, does not pay an allocation per
+ * call.  Calls are synchronous and thread-safe (serialised).
  *
  * Resident-state API
  * ------------------
@@ -118,6 +119,18 @@ int nlse_query(nlse_solver *h);
  * out3[0] = sum |U|^2 (mass / h^d), out3[1] = max Re U, out3[2] = number of
  * non-finite points (the reference drivers' blow-up test) */
 int nlse_diagnostics(nlse_solver *h, double *out3);
+/* discrete energy of the resident state, reduced on the GPU in double
+ * precision (no counterpart in the reference, whose drivers report the mass
+ * only: NLSEmagic1D_FRS.m:582-586).  With h^d the cell volume and nearest-
+ * neighbour pairs (p, q) along every axis, boundary points included:
+ *   out4[1] = h^d (a/h^2) sum_pairs |U_p - U_q|^2
+ *   out4[2] = h^d sum_p (V_p |U_p|^2 - (s/2) |U_p|^4)
+ *   out4[0] = out4[1] + out4[2]   (the energy E)
+ *   out4[3] = h^d sum_p |U_p|^2   (the mass)
+ * E is the Hamiltonian of the semi-discrete CD system (F = i[a Lap U +
+ * (s|U|^2 - V) U] is i dE/d conj(U) / h^d): with Dirichlet boundaries it is
+ * conserved up to the RK4 error.  Allocates nothing (the solver's workspace). */
+int nlse_energy(nlse_solver *h, double *out4);
 /* The reference drivers' full RK4 stability bound of the current state when
  * they pick k themselves (k = 0): NLSEmagic1D_FRS.m:233-275, NLSE2D.m:427-477,
  * NLSE3D.m:443-495.  With x = (h^2/a)(s|U|^2 - V) at every point and the
@@ -305,6 +318,11 @@ int nlse_copy_probe(long long bytes, int reps, double *ms);
 const char *nlse_last_error(void);
 /* number of HIP devices visible (0 when no GPU) */
 int nlse_device_count(void);
+/* number of device allocations the library has made since it was loaded
+ * (no counterpart in the reference): shows which calls allocate, e.g. that a
+ * chunk loop with per-chunk diagnostics allocates nothing after its first
+ * call */
+long long nlse_device_allocations(void);
 
 #ifdef __cplusplus
 }

@@ -129,6 +129,10 @@ def lib():
     L.nlse_set_step.argtypes = [C.c_void_p, C.c_double]
     L.nlse_diagnostics.restype = C.c_int
     L.nlse_diagnostics.argtypes = [C.c_void_p, _D]
+    L.nlse_energy.restype = C.c_int
+    L.nlse_energy.argtypes = [C.c_void_p, _D]
+    L.nlse_device_allocations.restype = C.c_longlong
+    L.nlse_device_allocations.argtypes = []
     L.nlse_batch_diagnostics.restype = C.c_int
     L.nlse_batch_diagnostics.argtypes = [C.c_void_p, _D]
     L.NLSE3D_VORTICITY_HIP.restype = C.c_int
@@ -358,6 +362,15 @@ class Solver:
         out = np.empty(3)
         _check(self._lib.nlse_diagnostics(self._h, _p(out, C.c_double)))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def energy(self):
+        """(E, gradient part, potential + nonlinear part, mass) of the resident
+        state, reduced on the GPU in FP64 (see nlse_energy): E = h^d [a/h^2
+        sum over neighbour pairs |U_p - U_q|^2 + sum (V|U|^2 - (s/2)|U|^4)],
+        the invariant of the semi-discrete CD flow with Dirichlet boundaries."""
+        out = np.empty(4)
+        _check(self._lib.nlse_energy(self._h, _p(out, C.c_double)))
+        return tuple(float(x) for x in out)
 
     def kfull(self):
         """(kfull, klin, k): the reference drivers' full RK4 stability bound of
@@ -711,3 +724,9 @@ def footprint_bytes(shape, method, precision):
 
 def device_count():
     return int(lib().nlse_device_count())
+
+
+def device_allocations():
+    """Device allocations the library has made since it was loaded (see
+    nlse_device_allocations)."""
+    return int(lib().nlse_device_allocations())

@@ -98,8 +98,8 @@ int build_tiles_t(nlse_batch* b)
     if (b->tile_prob) (void)hipFree(b->tile_prob);
     b->probs = nullptr;
     b->tile_prob = nullptr;
-    if (hipMalloc(&b->probs, sizeof(BatchProb2<T>) * P.size()) != hipSuccess ||
-        hipMalloc((void**)&b->tile_prob, sizeof(int) * tp.size()) != hipSuccess)
+    if (dev_malloc(&b->probs, sizeof(BatchProb2<T>) * P.size()) != hipSuccess ||
+        dev_malloc((void**)&b->tile_prob, sizeof(int) * tp.size()) != hipSuccess)
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     HIPCHK(hipMemcpy(b->probs, P.data(), sizeof(BatchProb2<T>) * P.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b->tile_prob, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice));
@@ -249,11 +249,11 @@ int nlse_batch_create(nlse_batch** out, int nprob, const int* N, const int* M, c
     const size_t eb = (precision == 2 ? 16 : 8) * (size_t)ntot;
     void** bufs[5] = {&b->U, &b->Y0, &b->Y1, &b->acc, &b->V};
     for (int i = 0; i < 5; ++i)
-        if (hipMalloc(bufs[i], i == 4 ? eb / 2 : eb) != hipSuccess) {
+        if (dev_malloc(bufs[i], i == 4 ? eb / 2 : eb) != hipSuccess) {
             nlse_batch_destroy(b);
             return fail(NLSE_ENOMEM, "hipMalloc failed");
         }
-    if (precision == 3 && hipMalloc(&b->Ulo, eb) != hipSuccess) {
+    if (precision == 3 && dev_malloc(&b->Ulo, eb) != hipSuccess) {
         nlse_batch_destroy(b);
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
@@ -361,7 +361,7 @@ int nlse_batch_diagnostics(nlse_batch* b, double* out)
 {
     if (!b || !out) return fail(NLSE_EINVAL, "null argument");
     double* d = nullptr;
-    HIPCHK(hipMalloc((void**)&d, sizeof(double) * (3 * 1024 + 3 * (size_t)b->nprob)));
+    HIPCHK(dev_malloc((void**)&d, sizeof(double) * (3 * 1024 + 3 * (size_t)b->nprob)));
     double* part = d + 3 * (size_t)b->nprob;
     const size_t eb = b->prec == 2 ? 16 : 8;
     hipError_t e = hipSuccess;
@@ -431,8 +431,8 @@ int nlse_batch_autotune(nlse_batch* b, int steps, int* best_fusion, int* best_ti
     if (!b || steps <= 0) return fail(NLSE_EINVAL, "bad argument");
     const size_t eb = (b->prec == 2 ? 16 : 8) * (size_t)b->ntot;
     void *saved = nullptr, *saved_lo = nullptr;
-    if (hipMalloc(&saved, eb) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
-    if (b->Ulo && hipMalloc(&saved_lo, eb) != hipSuccess) {
+    if (dev_malloc(&saved, eb) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    if (b->Ulo && dev_malloc(&saved_lo, eb) != hipSuccess) {
         (void)hipFree(saved);
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
@@ -444,7 +444,8 @@ int nlse_batch_autotune(nlse_batch* b, int steps, int* best_fusion, int* best_ti
     // rows per tile (strips: per segment) of each kernel family
     const int tl[3][3] = {{8, 16, 32}, {8, 12, 16}, {16, 32, 64}};
     const int gsave = b->gsteps;
-    int bf = b->fuse, bk = b->kern, bt = b->tile, bg = b->grid, rc = NLSE_OK;
+    const int of = b->fuse, ok = b->kern, ot = b->tile, og = b->grid;  // restored if a trial fails
+    int bf = of, bk = ok, bt = ot, bg = og, rc = NLSE_OK;
     double bms = 1e30;
     for (int kind = 0; kind < 3 && rc == NLSE_OK; ++kind)
         for (int x = 0; x < 3 && rc == NLSE_OK; ++x)
@@ -470,7 +471,10 @@ int nlse_batch_autotune(nlse_batch* b, int steps, int* best_fusion, int* best_ti
     (void)hipStreamSynchronize(b->stream);
     (void)hipFree(saved);
     if (saved_lo) (void)hipFree(saved_lo);
-    if (rc != NLSE_OK) return rc;
+    if (rc != NLSE_OK) {  // the batch keeps the configuration it had; the trial's error is reported
+        if (nlse_batch_set_kernel(b, ok, ot) == NLSE_OK) (void)nlse_batch_configure(b, of, ot, og, gsave);
+        return rc;
+    }
     rc = nlse_batch_set_kernel(b, bk, bt);
     if (rc == NLSE_OK) rc = nlse_batch_configure(b, bf, bt, bg, gsave);
     if (best_fusion) *best_fusion = bf;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -24,6 +25,14 @@ int set_error(int code, const std::string& msg)
 {
     g_err = msg;
     return code;
+}
+
+std::atomic<long long> g_allocs{0};
+
+hipError_t dev_malloc_raw(void** p, size_t bytes)
+{
+    ++g_allocs;
+    return hipMalloc(p, bytes);
 }
 
 void split_hilo(const double* x, float* hi, float* lo, long n)
@@ -66,11 +75,20 @@ struct nlse_solver {
     int tile;                 // 2D rows per tile / strip segment; 3D march rows; 1D steps per launch
     void* ubase;              // the buffer U was allocated in (U swaps with Y1 when fuse > 1)
     hipEvent_t ev0, ev1;
+    // reduction workspace (block partials and results of the diagnostics,
+    // energy, kfull and error-norm reductions, the vortex generators' radial
+    // profile), allocated with the solver: those calls allocate nothing
+    double* ws;
+    long ws_len;  // doubles
 };
 
 namespace {
 
 size_t elem_bytes(int prec) { return prec == 2 ? 16 : 8; }
+// workspace of a solver, in doubles: the widest reduction (6 values per
+// block, 1024 blocks, plus its result) and a vortex profile (at most twice
+// the grid's diagonal in points)
+long solver_ws_len(int N, int M, int L) { return 6 * 1024 + 16 + 4L * (N + M + L); }
 // precision codes: 2 double, 1 single, 3 single with a compensated (hi + lo) state
 bool is_double(const nlse_solver* h) { return h->prec == 2; }
 
@@ -352,34 +370,59 @@ IcGrid ic_grid(const nlse_solver* h, double xmin, double ymin, double zmin, doub
     q.ymin = ymin;
     q.zmin = zmin;
     q.h = hg;
+    q.ws = h->ws;
+    q.ws_len = h->ws_len;
     return q;
 }
 
-// ---- drop-in entry points: one cached solver per shape/precision
+// ---- drop-in entry points: cached solvers of the most recently used shapes
 
+// Up to DROPIN_SLOTS solvers (any shape, method and precision) keep their
+// device buffers between drop-in calls, so a caller that alternates between a
+// few problems does not reallocate; a new shape replaces the least recently
+// used solver.
+constexpr int DROPIN_SLOTS = 4;
 std::mutex g_cache_mu;
-nlse_solver* g_cache[2] = {nullptr, nullptr};  // indexed by precision-1
+nlse_solver* g_cache[DROPIN_SLOTS] = {};
+unsigned long g_cache_use[DROPIN_SLOTS] = {};
+unsigned long g_cache_clock = 0;
 
 nlse_solver* cached_solver(int dim, int N, int M, int L, int method, int prec, int BC, double s, double a,
                            double h2, double k, int* rc)
 {
-    nlse_solver*& slot = g_cache[prec - 1];
-    if (slot && (slot->dim != dim || slot->N != N || slot->M != M || slot->L != L || slot->method != method)) {
-        nlse_destroy(slot);
-        slot = nullptr;
+    if (dim == 1) M = L = 1;
+    if (dim == 2) L = 1;
+    *rc = validate(dim, N, M, L, method, prec, BC);  // a bad call evicts nothing
+    if (*rc != NLSE_OK) return nullptr;
+    int slot = -1;
+    for (int i = 0; i < DROPIN_SLOTS && slot < 0; ++i) {
+        const nlse_solver* c = g_cache[i];
+        if (c && c->dim == dim && c->N == N && c->M == M && c->L == L && c->method == method && c->prec == prec)
+            slot = i;
     }
-    if (!slot) {
-        *rc = nlse_create(&slot, dim, N, M, L, method, prec, BC, s, a, h2, k);
+    if (slot >= 0) {
+        set_params(g_cache[slot], BC, s, a, h2, k);
+    } else {
+        slot = 0;  // an empty slot, else the least recently used one
+        for (int i = 0; i < DROPIN_SLOTS; ++i) {
+            if (!g_cache[i]) {
+                slot = i;
+                break;
+            }
+            if (g_cache_use[i] < g_cache_use[slot]) slot = i;
+        }
+        if (g_cache[slot]) {
+            nlse_destroy(g_cache[slot]);
+            g_cache[slot] = nullptr;
+        }
+        *rc = nlse_create(&g_cache[slot], dim, N, M, L, method, prec, BC, s, a, h2, k);
         if (*rc != NLSE_OK) {
-            slot = nullptr;
+            g_cache[slot] = nullptr;
             return nullptr;
         }
-    } else {
-        *rc = validate(dim, N, M, L, method, prec, BC);
-        if (*rc != NLSE_OK) return nullptr;
-        set_params(slot, BC, s, a, h2, k);
     }
-    return slot;
+    g_cache_use[slot] = ++g_cache_clock;
+    return g_cache[slot];
 }
 
 template <typename T>
@@ -403,6 +446,8 @@ int take_steps(int dim, const T* Ur, const T* Ui, const T* V, T* Unr, T* Uni, do
 extern "C" {
 
 const char* nlse_last_error(void) { return g_err.c_str(); }
+
+long long nlse_device_allocations(void) { return g_allocs.load(); }
 
 int nlse_device_count(void)
 {
@@ -441,13 +486,18 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
     const size_t eb = elem_bytes(precision) * (size_t)h->n;
     void** bufs[4] = {&h->U, &h->Y0, &h->Y1, &h->acc};
     for (void** b : bufs)
-        if (hipMalloc(b, eb) != hipSuccess) { nlse_destroy(h); return fail(NLSE_ENOMEM, "hipMalloc failed"); }
-    if (hipMalloc(&h->V, eb / 2) != hipSuccess) { nlse_destroy(h); return fail(NLSE_ENOMEM, "hipMalloc failed"); }
-    if (dim == 3 && method == 2 && hipMalloc(&h->D, eb) != hipSuccess) {
+        if (dev_malloc(b, eb) != hipSuccess) { nlse_destroy(h); return fail(NLSE_ENOMEM, "hipMalloc failed"); }
+    if (dev_malloc(&h->V, eb / 2) != hipSuccess) { nlse_destroy(h); return fail(NLSE_ENOMEM, "hipMalloc failed"); }
+    if (dim == 3 && method == 2 && dev_malloc(&h->D, eb) != hipSuccess) {
         nlse_destroy(h);
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
-    if (precision == 3 && hipMalloc(&h->Ulo, eb) != hipSuccess) {
+    if (precision == 3 && dev_malloc(&h->Ulo, eb) != hipSuccess) {
+        nlse_destroy(h);
+        return fail(NLSE_ENOMEM, "hipMalloc failed");
+    }
+    h->ws_len = solver_ws_len(N, M, L);
+    if (dev_malloc((void**)&h->ws, sizeof(double) * (size_t)h->ws_len) != hipSuccess) {
         nlse_destroy(h);
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
@@ -485,7 +535,7 @@ void nlse_destroy(nlse_solver* h)
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     drop_graphs(h);
-    void* bufs[7] = {h->U, h->Y0, h->Y1, h->acc, h->D, h->V, h->Ulo};
+    void* bufs[8] = {h->U, h->Y0, h->Y1, h->acc, h->D, h->V, h->Ulo, h->ws};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -587,13 +637,35 @@ int nlse_profile_launches(nlse_solver* h, int steps, double* ms_launch, int* nla
 int nlse_diagnostics(nlse_solver* h, double* out3)
 {
     if (!h || !out3) return fail(NLSE_EINVAL, "null argument");
-    double* d = nullptr;
-    HIPCHK(hipMalloc((void**)&d, sizeof(double) * (3 * 1024 + 3)));
+    double* d = h->ws;  // result, then 3 * 1024 block partials
     hipError_t e = launch_diag(h->U, h->Ulo, is_double(h), h->n, d + 3, d, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(out3, d, 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(d);
     if (e != hipSuccess) return fail(NLSE_EHIP, std::string("diagnostics: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
+
+int nlse_energy(nlse_solver* h, double* out4)
+{
+    if (!h || !out4) return fail(NLSE_EINVAL, "null argument");
+    EnGrid g;
+    g.n = h->n;
+    g.dim = h->dim;
+    g.ext[0] = h->dim == 3 ? h->L : h->N;
+    g.ext[1] = h->dim == 3 ? h->N : h->M;
+    g.ext[2] = h->M;
+    g.s = h->s;
+    double* d = h->ws;  // result, then 3 * 1024 block partials
+    hipError_t e = launch_energy(h->U, h->Ulo, h->V, is_double(h), g, d + 3, d, h->stream);
+    double r[3];
+    if (e == hipSuccess) e = hipMemcpyAsync(r, d, sizeof(r), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("energy: ") + hipGetErrorString(e));
+    const double hd = std::pow(h->h2, 0.5 * h->dim);  // the cell volume h^d
+    out4[1] = hd * (h->a / h->h2) * r[0];
+    out4[2] = hd * r[1];
+    out4[0] = out4[1] + out4[2];
+    out4[3] = hd * r[2];
     return NLSE_OK;
 }
 
@@ -644,14 +716,12 @@ int nlse_kfull(nlse_solver* h, double* out3)
     g.c = h->h2 / h->a;
     g.s = h->s;
     g.BC = h->BC;
-    double* d = nullptr;
-    HIPCHK(hipMalloc((void**)&d, sizeof(double) * (6 * 1024 + 6)));
+    double* d = h->ws;  // result, then 6 * 1024 block partials
     hipError_t e = launch_kfull(h->U, h->Ulo, h->V, h->BC == BC_MSD ? h->acc : nullptr, is_double(h), g, d + 6, d,
                                 h->stream);
     double r[6];
     if (e == hipSuccess) e = hipMemcpyAsync(r, d, sizeof(r), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(d);
     if (e != hipSuccess) return fail(NLSE_EHIP, std::string("kfull: ") + hipGetErrorString(e));
     // Lmax = max over G of max over points |x - G|; Bmax by the BC
     double lmax = 0;
@@ -713,7 +783,7 @@ int nlse_vorticity(nlse_solver* h, double lh2, double veps, double* wx, double* 
     const size_t b = sizeof(double) * (size_t)h->n;
     int nout = 0;
     for (double* p : host) nout += p != nullptr;
-    if (nout && hipMalloc((void**)&buf, b * nout) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    if (nout && dev_malloc((void**)&buf, b * nout) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
     for (int o = 0, q = 0; o < 4; ++o)
         if (host[o]) dev[o] = buf + (size_t)h->n * q++;
     int rc = nlse_vorticity_device(h, lh2, veps, dev[0], dev[1], dev[2], dev[3]);
@@ -879,7 +949,7 @@ int nlse_autotune(nlse_solver* h, int steps, int* best_fusion, int* best_tile, d
     const size_t bytes = elem_bytes(h->prec) * (size_t)h->n;
     void* saved = nullptr;  // U, then the compensated low part when there is one
     const size_t sbytes = h->Ulo ? 2 * bytes : bytes;
-    if (hipMalloc(&saved, sbytes) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    if (dev_malloc(&saved, sbytes) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
     HIPCHK(hipMemcpyAsync(saved, h->U, bytes, hipMemcpyDeviceToDevice, h->stream));
     if (h->Ulo) HIPCHK(hipMemcpyAsync((char*)saved + bytes, h->Ulo, bytes, hipMemcpyDeviceToDevice, h->stream));
     // candidates: 2D every kernel family and tile height for 1, 2 and 4
@@ -941,8 +1011,8 @@ int nlse_copy_probe(long long bytes, int reps, double* ms)
     if (bytes <= 0 || bytes % 16 != 0 || reps <= 0 || !ms) return fail(NLSE_EINVAL, "bad argument");
     void *a = nullptr, *b = nullptr;
     hipEvent_t e0, e1;
-    HIPCHK(hipMalloc(&a, (size_t)bytes));
-    if (hipMalloc(&b, (size_t)bytes) != hipSuccess) {
+    HIPCHK(dev_malloc(&a, (size_t)bytes));
+    if (dev_malloc(&b, (size_t)bytes) != hipSuccess) {
         (void)hipFree(a);
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
@@ -1015,7 +1085,7 @@ int NLSE3D_VORTICITY_HIP(const double* Ur, const double* Ui, double lh2, double 
     const size_t b = sizeof(double) * (size_t)n;
     // U (2n), wx, wy, wz, W (4n); the outputs double as the upload staging area
     double* buf = nullptr;
-    if (hipMalloc((void**)&buf, b * 6) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
+    if (dev_malloc((void**)&buf, b * 6) != hipSuccess) return fail(NLSE_ENOMEM, "hipMalloc failed");
     hipStream_t st = nullptr;
     if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(buf);

@@ -3,6 +3,7 @@
 //   sum |U|^2        (the mass without its h^d factor; NLSEmagic1D_FRS.m:582-586)
 //   max Re(U)        (the drivers' blow-up test, NLSEmagic2D.m:285-288)
 //   non-finite count (the same test's isnan part)
+// and the discrete energy of the state (below).
 // Two deterministic passes: per-block partials in a fixed order, then one
 // block folds them, so repeated calls give identical bits.
 #include <cmath>
@@ -93,6 +94,105 @@ hipError_t launch_diag(const void* u, const void* ulo, bool dbl, long n, double*
         hipLaunchKernelGGL((diag_partial_kernel<float>), dim3(nb), dim3(256), 0, st, (const float2*)u,
                            (const float2*)ulo, n, part);
     hipLaunchKernelGGL(diag_fold_kernel, dim3(1), dim3(256), 0, st, part, nb, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Discrete energy.  With F = i [a Lap_CD U + (s|U|^2 - V) U] the semi-discrete
+// CD system is i U_t = dH/d conj(U) for
+//   H = a/h^2 sum_pairs |U_p - U_q|^2 + sum_p (V_p |U_p|^2 - (s/2) |U_p|^4)
+// (pairs: nearest neighbours along each axis, boundary points included), so H
+// is an invariant of the flow when the boundary values are held fixed
+// (Dirichlet).  The reference reports only the mass (NLSEmagic1D_FRS.m:582-586);
+// this is the energy counterpart.  Sums in double, state widened first (a
+// compensated single-precision state as hi + lo).
+
+template <typename T> __device__ __forceinline__ void load_wide(const c2<T>* u, const c2<T>* ulo, long p, double& re, double& im)
+{
+    const c2<T> v = u[p];
+    re = v.x;
+    im = v.y;
+    if (ulo) {
+        const c2<T> l = ulo[p];
+        re += (double)l.x;
+        im += (double)l.y;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void energy_partial_kernel(const c2<T>* __restrict__ u, const c2<T>* __restrict__ ulo,
+                                                             const T* __restrict__ V, EnGrid g, double* __restrict__ part)
+{
+    __shared__ double s_g[256], s_p[256], s_m[256];
+    double gsum = 0.0, psum = 0.0, msum = 0.0;
+    long stride[3];
+    stride[g.dim - 1] = 1;
+    for (int d = g.dim - 2; d >= 0; --d) stride[d] = stride[d + 1] * g.ext[d + 1];
+    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < g.n; p += (long)gridDim.x * 256) {
+        double re, im;
+        load_wide<T>(u, ulo, p, re, im);
+        const double m2 = re * re + im * im;
+        msum += m2;
+        psum += (double)V[p] * m2 - 0.5 * g.s * m2 * m2;
+        long r = p;
+        for (int d = g.dim - 1; d >= 0; --d) {
+            const int c = (int)(r % g.ext[d]);
+            r /= g.ext[d];
+            if (c + 1 < g.ext[d]) {  // the pair (p, p + e_d)
+                double qr, qi;
+                load_wide<T>(u, ulo, p + stride[d], qr, qi);
+                const double dr = re - qr, di = im - qi;
+                gsum += dr * dr + di * di;
+            }
+        }
+    }
+    s_g[threadIdx.x] = gsum;
+    s_p[threadIdx.x] = psum;
+    s_m[threadIdx.x] = msum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s_g[threadIdx.x] += s_g[threadIdx.x + w];
+            s_p[threadIdx.x] += s_p[threadIdx.x + w];
+            s_m[threadIdx.x] += s_m[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x + 0] = s_g[0];
+        part[3 * blockIdx.x + 1] = s_p[0];
+        part[3 * blockIdx.x + 2] = s_m[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void sum3_fold_kernel(const double* __restrict__ part, int nb, double* __restrict__ out)
+{
+    __shared__ double s[3][256];
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < nb; b += 256)
+        for (int i = 0; i < 3; ++i) a[i] += part[3 * b + i];
+    for (int i = 0; i < 3; ++i) s[i][threadIdx.x] = a[i];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int i = 0; i < 3; ++i) s[i][threadIdx.x] += s[i][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int i = 0; i < 3; ++i) out[i] = s[i][0];
+}
+
+hipError_t launch_energy(const void* u, const void* ulo, const void* V, bool dbl, const EnGrid& g, double* part,
+                         double* out, hipStream_t st)
+{
+    const int nb = (int)std::min<long>(DIAG_BLOCKS, (g.n + 255) / 256);
+    if (dbl)
+        hipLaunchKernelGGL((energy_partial_kernel<double>), dim3(nb), dim3(256), 0, st, (const double2*)u,
+                           (const double2*)nullptr, (const double*)V, g, part);
+    else
+        hipLaunchKernelGGL((energy_partial_kernel<float>), dim3(nb), dim3(256), 0, st, (const float2*)u,
+                           (const float2*)ulo, (const float*)V, g, part);
+    hipLaunchKernelGGL(sum3_fold_kernel, dim3(1), dim3(256), 0, st, part, nb, out);
     return hipGetLastError();
 }
 

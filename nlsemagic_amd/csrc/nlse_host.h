@@ -14,6 +14,12 @@ namespace nlse {
 // record the message returned by nlse_last_error() (thread-local) and return code
 int set_error(int code, const std::string& msg);
 
+// Every device allocation of the library goes through dev_malloc, which
+// counts it (nlse_device_allocations reports the count), so tests can show
+// which calls allocate.
+hipError_t dev_malloc_raw(void** p, size_t bytes);
+template <typename T> inline hipError_t dev_malloc(T** p, size_t bytes) { return dev_malloc_raw((void**)p, bytes); }
+
 #define HIPCHK(x)                                                                                      \
     do {                                                                                               \
         hipError_t e_ = (x);                                                                           \
@@ -140,6 +146,17 @@ struct KfGrid {
 // F (the stage-1 RHS of the state, same precision) is read for MSD only
 hipError_t launch_kfull(const void* u, const void* ulo, const void* V, const void* F, bool dbl, const KfGrid& g,
                         double* part, double* out, hipStream_t st);
+// The discrete energy of a state, in double precision: over nearest-
+// neighbour pairs (p, q) along every axis G = sum |U_p - U_q|^2, and
+// P = sum_p (V_p |U_p|^2 - (s/2) |U_p|^4), S = sum_p |U_p|^2, into out[0..2]
+// (device); `part` holds 3 * DIAG_BLOCKS doubles.  ext: extents, slowest first.
+struct EnGrid {
+    long n;
+    int dim, ext[3];
+    double s;
+};
+hipError_t launch_energy(const void* u, const void* ulo, const void* V, bool dbl, const EnGrid& g, double* part,
+                         double* out, hipStream_t st);
 // 3D vorticity of a state (L x N x M, k fastest) in double precision into
 // wx, wy, wz, W (device pointers, any may be null); ulo: low part of a
 // compensated single-precision state (or null).  No scratch memory.
@@ -153,6 +170,8 @@ hipError_t launch_vorticity3d(const c2<T>* u, const c2<T>* ulo, int L, int N, in
 struct IcGrid {
     int dim, N, M, L;
     double xmin, ymin, zmin, h;
+    double* ws;   // device workspace of the solver (reduction partials, vortex profile)
+    long ws_len;  // its length in doubles
 };
 int vortex_profile(int m, double OM, double a, double s, double hr, int n, int ansatz, int optimize, double tol,
                    double* out);

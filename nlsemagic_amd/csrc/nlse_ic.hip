@@ -574,15 +574,14 @@ int ic_exact_error(const c2<T>* u, const c2<T>* ulo, const IcGrid& q, int kind, 
     const GridC g = make_grid(q);
     Exact e{kind, par[0], par[1], par[2], par[3], par[4]};
     const int nb = ic_blocks(g.n);
-    double* d = nullptr;
-    HIPCHK(hipMalloc((void**)&d, sizeof(double) * (6 * nb + 6)));
+    if (!q.ws || q.ws_len < 6L * nb + 6) return set_error(NLSE_EINVAL, "exact error: workspace too small");
+    double* d = q.ws;  // result, then the block partials
     hipLaunchKernelGGL((exact_err_partial<T>), dim3(nb), dim3(256), 0, st, u, ulo, g, e, t, d + 6);
     hipLaunchKernelGGL(exact_err_fold, dim3(1), dim3(256), 0, st, d + 6, nb, g.n, d);
     hipError_t err = hipGetLastError();
     if (err == hipSuccess) err = hipMemcpyAsync(out6, d, 6 * sizeof(double), hipMemcpyDeviceToHost, st);
     if (err == hipSuccess) err = hipStreamSynchronize(st);
     (void)hipStreamSynchronize(st);
-    (void)hipFree(d);
     if (err != hipSuccess) return set_error(NLSE_EHIP, std::string("exact error: ") + hipGetErrorString(err));
     return NLSE_OK;
 }
@@ -594,8 +593,15 @@ static int run_vortex(c2<T>* u, c2<T>* ulo, const IcGrid& q, Vortex v, const std
 {
     const GridC g = make_grid(q);
     const int nb = ic_blocks(g.n);
-    double* d = nullptr;
-    HIPCHK(hipMalloc((void**)&d, sizeof(double) * (3 * nb + 2 + prof.size())));
+    // partials, amplitude and profile in the solver's workspace (sized for a
+    // profile of twice the grid diagonal); a longer profile gets its own buffer
+    const long need = 3L * nb + 2 + (long)prof.size();
+    double* own = nullptr;
+    double* d = q.ws;
+    if (!d || q.ws_len < need) {
+        HIPCHK(dev_malloc((void**)&own, sizeof(double) * (size_t)need));
+        d = own;
+    }
     double* part = d;
     double* amp = d + 3 * nb;
     double* dprof = amp + 2;
@@ -615,7 +621,7 @@ static int run_vortex(c2<T>* u, c2<T>* ulo, const IcGrid& q, Vortex v, const std
     }
     if (err == hipSuccess) err = hipStreamSynchronize(st);
     (void)hipStreamSynchronize(st);
-    (void)hipFree(d);
+    if (own) (void)hipFree(own);
     if (err != hipSuccess) return set_error(NLSE_EHIP, std::string("vortex: ") + hipGetErrorString(err));
     return NLSE_OK;
 }
