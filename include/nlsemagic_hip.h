@@ -131,6 +131,20 @@ int nlse_diagnostics(nlse_solver *h, double *out3);
  * (s|U|^2 - V) U] is i dE/d conj(U) / h^d): with Dirichlet boundaries it is
  * conserved up to the RK4 error.  Allocates nothing (the solver's workspace). */
 int nlse_energy(nlse_solver *h, double *out4);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the reduction of
+ * nlse_diagnostics over a state the caller holds on the GPU.
+ *   U     n interleaved complex values on the current GPU (double for
+ *         precision 2, float for 1 and 3)
+ *   Ulo   low part of a compensated (precision 3) state, same layout, or NULL
+ *   out3  device memory for 3 doubles: sum |U|^2, max Re U, non-finite count
+ *   stream  a hipStream_t (NULL: the null stream); the call only enqueues
+ * Allocates nothing: the block partials (3 x 1024 doubles) live in a device
+ * array of the library, one copy per GPU, located on the first call on that
+ * GPU.  Calls that overlap on one GPU share that array and race: serialise
+ * them (one stream, or events).  n: 1 .. 2^31-1, else NLSE_EINVAL. */
+int nlse_diagnostics_noalloc(const void *U, const void *Ulo, long long n, int precision, double *out3,
+                             void *stream);
 /* The reference drivers' full RK4 stability bound of the current state when
  * they pick k themselves (k = 0): NLSEmagic1D_FRS.m:233-275, NLSE2D.m:427-477,
  * NLSE3D.m:443-495.  With x = (h^2/a)(s|U|^2 - V) at every point and the

@@ -133,6 +133,8 @@ def lib():
     L.nlse_energy.argtypes = [C.c_void_p, _D]
     L.nlse_device_allocations.restype = C.c_longlong
     L.nlse_device_allocations.argtypes = []
+    L.nlse_diagnostics_noalloc.restype = C.c_int
+    L.nlse_diagnostics_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p]
     L.nlse_batch_diagnostics.restype = C.c_int
     L.nlse_batch_diagnostics.argtypes = [C.c_void_p, _D]
     L.NLSE3D_VORTICITY_HIP.restype = C.c_int

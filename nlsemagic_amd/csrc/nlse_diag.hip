@@ -7,12 +7,20 @@
 // Two deterministic passes: per-block partials in a fixed order, then one
 // block folds them, so repeated calls give identical bits.
 #include <cmath>
+#include <mutex>
 
 #include "nlse_host.h"
 
 namespace nlse {
 
 constexpr int DIAG_BLOCKS = 1024;
+
+// The block partials of nlse_diagnostics_noalloc: a device array of the
+// library (one copy per GPU), so that entry point allocates nothing.  Host
+// code never passes `scratch` itself (a host-side stand-in), only the device
+// address hipGetSymbolAddress returns for the current GPU.
+constexpr int NOALLOC_PARTIALS = 3 * DIAG_BLOCKS;
+static __device__ __align__(256) double scratch[NOALLOC_PARTIALS];
 
 template <typename T>
 __global__ __launch_bounds__(256) void diag_partial_kernel(const c2<T>* __restrict__ u, const c2<T>* __restrict__ ulo,
@@ -346,3 +354,45 @@ hipError_t launch_kfull(const void* u, const void* ulo, const void* V, const voi
 }
 
 }  // namespace nlse
+
+// ---------------------------------------------------------------------------
+// nlse_diagnostics_noalloc: the reduction of nlse_diagnostics over a state the
+// caller holds on the GPU, with the block partials in `scratch`.
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};  // device address of `scratch`, per GPU
+}  // namespace
+
+extern "C" int nlse_diagnostics_noalloc(const void* U, const void* Ulo, long long n, int precision, double* out3,
+                                        void* stream)
+{
+    using namespace nlse;
+    if (!U || !out3) return set_error(NLSE_EINVAL, "diagnostics_noalloc: null U or out3");
+    if (precision < 1 || precision > 3)
+        return set_error(NLSE_EINVAL, "diagnostics_noalloc: precision must be 1, 2 or 3");
+    if (precision != 3 && Ulo) return set_error(NLSE_EINVAL, "diagnostics_noalloc: Ulo is for precision 3 only");
+    if (n < 1 || n > (1LL << 31) - 1)
+        return set_error(NLSE_EINVAL, "diagnostics_noalloc: n must be in 1 .. 2^31-1 points");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0)
+        return set_error(NLSE_EHIP, "diagnostics_noalloc: no current HIP device");
+    if (dev >= NOALLOC_MAX_DEVICES) return set_error(NLSE_EINVAL, "diagnostics_noalloc: device index above 63");
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_dev[dev]) {  // first call on this GPU (the current device): its copy's address
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return set_error(NLSE_EHIP, std::string("diagnostics_noalloc: hipGetSymbolAddress: ") +
+                                                hipGetErrorString(e));
+            g_scratch_dev[dev] = (double*)p;
+        }
+        part = g_scratch_dev[dev];
+    }
+    const hipError_t e = launch_diag(U, Ulo, precision == 2, (long)n, part, out3, (hipStream_t)stream);
+    if (e != hipSuccess) return set_error(NLSE_EHIP, std::string("diagnostics_noalloc: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
