@@ -28,7 +28,12 @@ from oracle import oracle as O  # noqa: E402
 
 
 def cases():
-    """(name, shape, method, precision, BC, s, a, h2, k, chunk, seed)"""
+    """(name, shape, method, precision, BC, s, a, h2, k, chunk, seed, zero)
+
+    zero: None, or (value, points) -- U set to `value` at the listed points
+    before the run: MSD partners of zero modulus, where the reference divides
+    by |U|^2 = 0 unguarded (NLSEmagic2D_CPU.c:64-68, :297-303) and produces
+    NaN, and a modulus whose square underflows to 0 (x/0 = inf)."""
     out = []
     seed = 100
     for method in (1, 2):
@@ -42,7 +47,20 @@ def cases():
                     seed += 1
                     out.append((f"2d_m{method}_p{prec}_bc{BC}_{shape[0]}x{shape[1]}", shape, method, prec, BC,
                                 -1.0 if BC != 1 else 0.7, 0.8, 0.01, 0.0015, 5, seed))
-    return out
+    # MSD with zero-modulus partners (chunks of one step, so the non-finite
+    # values have not yet reached every point)
+    for method in (1, 2):
+        for prec in (2, 1):
+            for tag, val in (("zero", 0.0), ("tiny", 1e-170)):
+                if prec == 1 and tag == "tiny":
+                    continue  # 1e-170 is already 0 in single precision
+                seed += 1
+                out.append((f"1d_m{method}_p{prec}_bc2_{tag}", (40,), method, prec, 2, -1.0, 0.9, 0.04, 0.006, 1,
+                            seed, (val, [(1,), (38,)])))
+                seed += 1
+                out.append((f"2d_m{method}_p{prec}_bc2_{tag}_23x19", (23, 19), method, prec, 2, -1.0, 0.8, 0.01,
+                            0.0015, 1, seed, (val, [(1, 7), (11, 1), (21, 12), (9, 17), (1, 1), (21, 17)])))
+    return [c if len(c) == 12 else c + (None,) for c in out]
 
 
 def inputs(shape, seed):
@@ -57,8 +75,11 @@ def main():
         raise SystemExit("oracle/_ref not built: run `make -C oracle ref` with /root/reference present")
     arrays = {}
     manifest = []
-    for (name, shape, method, prec, BC, s, a, h2, k, chunk, seed) in cases():
+    for (name, shape, method, prec, BC, s, a, h2, k, chunk, seed, zero) in cases():
         U, V = inputs(shape, seed)
+        if zero is not None:
+            for pt in zero[1]:
+                U[pt] = zero[0]
         out = O.take_steps(U, V, s, a, h2, BC, chunk, k, method, precision=prec, use_ref=True)
         arrays[f"{name}__U"] = U
         arrays[f"{name}__V"] = V

@@ -261,6 +261,71 @@ def test_3d_rhs_equals_shift_operator_form(oracle, method):
     assert np.allclose(got, exp, rtol=1e-11, atol=1e-11 * np.max(np.abs(exp)))
 
 
+def _msd_partner_d(D, U, V, s, a, partner, sign=1.0):
+    """MSD boundary D (NLSE3D_TAKE_STEPS_2SHOC.c:100-190): from the partner's
+    D and U, A = Re(D_q conj U_q)/|U_q|^2, D_b = (A + (n_q - n_b)/a) U_b with
+    n = s|U|^2 - V; partner(b) gives the partner index of boundary point b
+    (sign = -1 plants a sign mistake in n_q - n_b)."""
+    out = D.copy()
+    for b in _boundary_points(U.shape):
+        q = partner(b, U.shape)
+        A = (D[q] * np.conj(U[q])).real / abs(U[q]) ** 2
+        nq, nb = s * abs(U[q]) ** 2 - V[q], s * abs(U[b]) ** 2 - V[b]
+        out[b] = (A + sign * (nq - nb) / a) * U[b]
+    return out
+
+
+def _clamped(b, shape):
+    return tuple(min(max(i, 1), n - 2) for i, n in zip(b, shape))
+
+
+def _second_layer(b, shape):
+    """A planted mistake: the partner two layers in."""
+    return tuple(min(max(i, 2), n - 3) for i, n in zip(b, shape))
+
+
+@pytest.mark.parametrize("BC", [2, 3])
+def test_3d_2shoc_boundary_d_shift_operator_form(oracle, BC):
+    """2SHOC with MSD (D on the boundary from the partner with every
+    coordinate clamped into [1, n-2]) and Lap0 (D = 0 on the boundary): the
+    interior F of the shift-operator form equals the oracle's, which pins the
+    3D boundary D on every face (each interior point next to a face reads the
+    face's D).  Edge and corner D are read by no point: the 2SHOC F takes D
+    only from the six axis neighbours and the MSD boundary F only from
+    interior partners, so their partner rule cannot change a result.
+    Planted mistakes (a partner two layers in, a flipped n_q - n_b, a
+    Dirichlet-style D on a Lap0 boundary) all fail."""
+    shape = (6, 7, 8)
+    rng = np.random.default_rng(8 + BC)
+    U = 1 + 0.3 * rng.uniform(-1, 1, shape) + 0.3j * rng.uniform(-1, 1, shape)
+    V = rng.uniform(-1, 1, shape)
+    s, a, h2 = -0.6, 0.85, 0.05
+    inner = np.zeros(shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    T = [lambda X, ax=ax: _shift(X, ax, 1) + _shift(X, ax, -1) for ax in range(3)]
+    lap = lambda X: (T[0](X) + T[1](X) + T[2](X) - 6 * X) / h2
+    nl = (s * np.abs(U) ** 2 - V) * U
+    edges = T[0](T[1](U)) + T[1](T[2](U)) + T[2](T[0](U))
+
+    def interior_f(D):
+        Lu = D / 3 - h2 / 12 * lap(D) + (edges - 12 * U) / (6 * h2)
+        return 1j * (a * Lu + nl)
+
+    got = oracle.rhs(U, V, s, a, h2, BC, 2)
+    Din = np.where(inner, lap(U), 0)
+    if BC == 2:
+        right = _msd_partner_d(Din, U, V, s, a, _clamped)
+        wrongs = [_msd_partner_d(Din, U, V, s, a, _second_layer), _msd_partner_d(Din, U, V, s, a, _clamped, -1.0)]
+    else:
+        right = Din
+        wrongs = [np.where(inner, Din, (V - s * np.abs(U) ** 2) * U / a)]
+    exp = interior_f(right)
+    tol = 1e-11 * np.max(np.abs(exp[inner]))
+    assert np.max(np.abs(got[inner] - exp[inner])) < tol
+    for wrong in wrongs:
+        assert np.max(np.abs(got[inner] - interior_f(wrong)[inner])) > 1e3 * tol
+
+
 def test_single_precision_tracks_double(oracle):
     rng = np.random.default_rng(2)
     for shape in ((64,), (20, 24), (8, 10, 12)):
