@@ -50,6 +50,7 @@ class Request:
     steps: int
     make_state: Optional[Callable[[], tuple]] = None  # -> (U0, V) host arrays, uploaded
     init: Optional[Callable[[Solver], None]] = None   # builds the state on the device instead
+    make_V: Optional[Callable[[], np.ndarray]] = None  # with init: a potential uploaded after it (else V = 0)
     admitted: Optional[float] = None
     finished: Optional[float] = None
     result: Optional[np.ndarray] = None
@@ -72,6 +73,8 @@ class Request:
         request's parameters (init) or from host arrays (make_state)."""
         if self.init is not None:
             self.init(sol)
+            if self.make_V is not None:
+                sol.set_state(sol.state(), self.make_V())
         else:
             U0, V = self.make_state()
             sol.set_state(U0, V)
@@ -83,7 +86,7 @@ class Request:
         sol = Solver(self.shape, self.method, 2, self.BC, self.s, self.a, self.h2, self.k)
         try:
             self.init(sol)
-            return sol.state(), np.zeros(self.shape)
+            return sol.state(), (self.make_V() if self.make_V is not None else np.zeros(self.shape))
         finally:
             sol.close()
 
@@ -218,7 +221,7 @@ def plan_chunks(endtw: float, k: float, numframes: int):
     return chunk_size, numframes, steps, endt
 
 
-def request_mix(n: int, qps: float, seed: int = 2) -> List[Request]:
+def request_mix(n: int, qps: float, seed: int = 2, variety: bool = False) -> List[Request]:
     """The serving workload: Poisson arrivals at `qps`, uniform over
     {1D, 2D, 3D} x {CD, 2SHOC} x {compensated FP32, FP64}; sizes log-uniform
     in 1D [4096, 65536] points, 2D [128, 1024]^2, 3D [64, 192]^3; steps
@@ -227,8 +230,13 @@ def request_mix(n: int, qps: float, seed: int = 2) -> List[Request]:
     size, steps.  s = -1, a = 1, V = 0, Dirichlet boundaries, k = 0.8 klin;
     h = 0.1 (1D, 2D) or 0.2 (3D); the initial state arrives as parameters
     and is built on the device (default_ic): a dark soliton (1D), a dark
-    vortex (2D) or a dark vortex ring of radius a quarter of the box (3D)."""
+    vortex (2D) or a dark vortex ring of radius a quarter of the box (3D).
+    variety: per request, from a second stream seeded with (seed, 1), the
+    boundary (Dirichlet or MSD, even odds) and, with even odds, a harmonic
+    trap V = -omega^2 |x|^2 (omega ~ U[0.05, 0.1], make_V) uploaded after the
+    state is built; the first stream's draws are unchanged."""
     rng = np.random.default_rng(seed)
+    rng2 = np.random.default_rng([seed, 1])
     t = 0.0
     out = []
     lu = lambda lo, hi: float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
@@ -247,9 +255,22 @@ def request_mix(n: int, qps: float, seed: int = 2) -> List[Request]:
             shape = (side, side, side)
         steps = int(round(lu(100, 20000)))
         h = 0.1 if dim < 3 else 0.2
-        out.append(Request(i, t, shape, method, precision, 1, -1.0, 1.0, h * h, auto_k(dim, method, h, 1.0),
-                           steps, init=default_ic(shape, h)))
+        BC, make_V = 1, None
+        if variety:
+            BC = (1, 2)[int(rng2.integers(0, 2))]
+            if rng2.uniform() < 0.5:
+                make_V = trap(shape, h, float(rng2.uniform(0.05, 0.1)))
+        out.append(Request(i, t, shape, method, precision, BC, -1.0, 1.0, h * h, auto_k(dim, method, h, 1.0),
+                           steps, init=default_ic(shape, h), make_V=make_V))
     return out
+
+
+def trap(shape, h, omega):
+    """The harmonic trap V = -omega^2 |x|^2 on the centred grid of spacing h."""
+    def make_V():
+        axes = [h * (np.arange(n) - (n - 1) / 2) for n in shape]
+        return -omega * omega * sum(g * g for g in np.meshgrid(*axes, indexing="ij"))
+    return make_V
 
 
 def default_ic(shape, h):

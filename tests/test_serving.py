@@ -92,3 +92,27 @@ def test_admission_respects_the_memory_budget():
     assert srv.peak_bytes <= 1.5 * biggest
     assert all(r.result is not None and np.all(np.isfinite(r.result)) for r in reqs)
     assert stats["requests"] == len(reqs)
+
+
+@pytest.mark.gpu
+def test_config_sizes_with_msd_and_traps_alone_and_in_the_mix():
+    """Requests at the serving configuration's sizes (1D 4k-64k points, 2D
+    128^2-1024^2, 3D 64^3-192^3, CD and 2SHOC, FP64 and compensated FP32),
+    with MSD boundaries and harmonic traps in the mix (steps capped at 240 to
+    bound the run): each result is bit for bit the result of the request
+    run alone."""
+    import nlsemagic_amd as nm
+    reqs = S.request_mix(10, qps=100.0, seed=2, variety=True)
+    for r in reqs:
+        r.steps = min(r.steps, 240)
+    assert {r.BC for r in reqs} == {1, 2} and {len(r.shape) for r in reqs} == {1, 2, 3}
+    stats = S.Server(max_active=16, chunk_steps=64).run(reqs)
+    assert stats["requests"] == len(reqs)
+    for r in reqs:
+        print("CASE", r.rid, r.shape, r.method, r.precision, r.BC, r.steps, flush=True)
+        sol = nm.Solver(r.shape, r.method, r.precision, r.BC, r.s, r.a, r.h2, r.k)
+        r.initialize(sol)
+        sol.step(r.steps)
+        alone = sol.state()
+        sol.close()
+        assert np.array_equal(alone.view(np.uint64), r.result.view(np.uint64)), r.rid
