@@ -351,6 +351,54 @@ def run_2d_matrix(args):
     print(json.dumps(line))
 
 
+def run_parity_2d(args):
+    """The 2D configuration's whole run against the reference CPU integrator
+    (oracle/_ref, src/NLSEmagic2D/NLSEmagic2D_CPU.c, one thread): 1024^2, MSD,
+    the dark vortex + 1e-3 noise (seed 0), 5000 steps in chunks of 50, CD at
+    the stated k = 0.3 h^2 and 2SHOC at the drivers' k = 0.2121 h^2 (0.3 h^2
+    overflows 2SHOC).  FP64 on the GPU against the CPU's FP64 result (bits,
+    relative L2), FP32 and compensated FP32 against it (relative L2), and the
+    mass sum |U|^2 h^2 drift of each.  About 4 minutes of CPU time."""
+    import nlsemagic_amd as nm
+    from oracle import oracle as O
+    U, V = initial_state()
+    steps, chunk = args.steps if args.steps != 40 else 5000, CHUNK
+    use_ref = O.ref(2) is not None
+    mass = lambda A: float(np.sum(np.abs(A) ** 2)) * H * H
+    m0 = mass(U)
+    out = {}
+    for method, kf in ((1, 0.3), (2, K_RULE)):
+        k = kf * H * H
+        name = "CD" if method == 1 else "2SHOC"
+        t0 = time.perf_counter()
+        W = U
+        for done in range(0, steps, chunk):
+            W = O.take_steps(W, V, -1.0, 1.0, H * H, 2, chunk, k, method, use_ref=use_ref)
+            if (done + chunk) % 500 == 0:
+                print(f"parity-2d: {name} CPU {done + chunk}/{steps} steps", file=sys.stderr, flush=True)
+        cpu_s = time.perf_counter() - t0
+        res = {"k_over_h2": kf, "cpu_seconds_1thread": cpu_s, "cpu_kind": "reference" if use_ref else "port",
+               "mass_initial": m0, "mass_drift_cpu": mass(W) / m0 - 1}
+        for prec, tag in ((2, "fp64"), (1, "fp32"), (3, "fp32c")):
+            sol = nm.Solver((N, M), method, prec, 2, -1.0, 1.0, H * H, k)
+            sol.set_graph(chunk)
+            sol.set_state(U, V)
+            sol.step(steps)
+            G = sol.state()
+            sol.close()
+            r = {"rel_l2_vs_cpu_fp64": float(np.linalg.norm(G - W) / np.linalg.norm(W)),
+                 "max_abs_diff": float(np.max(np.abs(G - W))), "mass_drift": mass(G) / m0 - 1,
+                 "finite": bool(np.all(np.isfinite(G)))}
+            if prec == 2:
+                r["bit_identical"] = bool(np.array_equal(G.view(np.uint64), W.view(np.uint64)))
+            res[tag] = r
+            print(f"parity-2d: {name} {tag} done", file=sys.stderr, flush=True)
+        out[name] = res
+    print(json.dumps({"workload": "parity-2d: 1024x1024 MSD dark vortex + 1e-3 noise (seed 0), "
+                                  f"{steps} steps in chunks of {chunk}, GPU vs the reference CPU integrator",
+                      "steps": steps, "cpu_host": host_cpu(), "results": out}))
+
+
 def vortex_rings_3d(n=256, L=25.6, nrings=4, radius=8.0, seed=0):
     """Product of dark vortex rings tanh(d/sqrt 2) e^{+-i phi} on [-L, L)^3:
     d is the distance to the ring core, phi the poloidal angle around it.
@@ -644,7 +692,8 @@ def main():
     ap.add_argument("--qps", type=float, default=4.0, help="serving: Poisson arrival rate")
     ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
     ap.add_argument("--workload", default="2d",
-                    choices=("2d", "2d-matrix", "batched", "1d", "3d", "3d-counters", "serving", "vorticity"),
+                    choices=("2d", "2d-matrix", "batched", "1d", "3d", "3d-counters", "serving", "vorticity",
+                             "parity-2d"),
                     help="2d: the 1024^2 FP64 2SHOC headline; batched: 512 uneven problems, FP32c vs FP64")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (HBM counter calibration) and exit")
@@ -661,6 +710,9 @@ def main():
         sys.exit(2)
     if args.workload == "serving":
         run_serving(args)
+        return
+    if args.workload == "parity-2d":
+        run_parity_2d(args)
         return
     if args.workload in ("1d", "3d"):
         run_grid(args)
