@@ -323,6 +323,23 @@ int nlse_batch_configure(nlse_batch *b, int stages_per_launch, int tile_rows, in
 int nlse_batch_autotune(nlse_batch *b, int steps, int *best_fusion, int *best_tile, int *best_grid,
                         double *best_ms);
 
+/* Groups of resident 2D solvers (no counterpart in the reference, which
+ * solves one problem per call): the members (2D, one method and precision;
+ * any sizes, parameters and boundaries) are advanced together by the batched
+ * kernels, one persistent launch per pair of RK4 stages over the tiles of
+ * every member, each in its own buffers.  Results are bit-identical to
+ * stepping each member alone.  nlse_group_create waits for the members'
+ * queued work; nlse_group_step enqueues on the group's own stream (returns
+ * at once); nlse_group_done returns 1 when the group's work has finished, 0
+ * while it runs (a negative NLSE_E* code on error); nlse_group_destroy waits
+ * for it and hands the members back (their state advanced).  While a group
+ * exists, its members must not be stepped, read or written on their own. */
+typedef struct nlse_group nlse_group;
+int nlse_group_create(nlse_group **g, nlse_solver *const *members, int n);
+int nlse_group_step(nlse_group *g, int steps);
+int nlse_group_done(nlse_group *g);
+void nlse_group_destroy(nlse_group *g);
+
 /* measurement helper: mean device ms of a plain streaming copy of `bytes`
  * (multiple of 16) between two fresh device buffers, over `reps` launches;
  * gives the HBM counters' reading for a known byte count */

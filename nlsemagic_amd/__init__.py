@@ -30,7 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnlsemagic_hip.so")
 
 __all__ = [
-    "lib", "Solver", "Batch", "HipError", "DeviceArray", "rmc_vorticity3D", "vortex_profile",
+    "lib", "Solver", "Batch", "Group", "HipError", "DeviceArray", "rmc_vorticity3D", "vortex_profile",
     "NLS_add_vortex", "NLSE_add_vortex_line", "NLSE_add_vortex_ring",
     "NLSEmagic1D_Take_Steps_GPU", "NLSEmagic2D_Take_Steps_GPU", "NLSEmagic3D_Take_Steps_GPU",
 ]
@@ -133,6 +133,14 @@ def lib():
     L.nlse_energy.argtypes = [C.c_void_p, _D]
     L.nlse_device_allocations.restype = C.c_longlong
     L.nlse_device_allocations.argtypes = []
+    L.nlse_group_create.restype = C.c_int
+    L.nlse_group_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int]
+    L.nlse_group_step.restype = C.c_int
+    L.nlse_group_step.argtypes = [C.c_void_p, C.c_int]
+    L.nlse_group_done.restype = C.c_int
+    L.nlse_group_done.argtypes = [C.c_void_p]
+    L.nlse_group_destroy.restype = None
+    L.nlse_group_destroy.argtypes = [C.c_void_p]
     L.nlse_diagnostics_noalloc.restype = C.c_int
     L.nlse_diagnostics_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p, C.c_void_p]
     L.nlse_batch_diagnostics.restype = C.c_int
@@ -501,6 +509,41 @@ class Solver:
 
     def set_graph(self, steps_per_graph):
         _check(self._lib.nlse_set_graph(self._h, int(steps_per_graph)))
+
+
+class Group:
+    """Resident 2D solvers (one method and precision, any sizes) advanced
+    together by shared launches (the nlse_group_* ABI); each member's result
+    is bit for bit its result alone.  While the group is open its members
+    must not be used on their own; close() waits and hands them back."""
+
+    def __init__(self, solvers):
+        self._lib = lib()
+        self.members = list(solvers)
+        arr = (C.c_void_p * len(self.members))(*[s._h.value for s in self.members])
+        self._h = C.c_void_p()
+        _check(self._lib.nlse_group_create(C.byref(self._h), arr, len(self.members)))
+
+    def step(self, steps):
+        """Enqueue `steps` RK4 steps of every member (returns at once)."""
+        _check(self._lib.nlse_group_step(self._h, int(steps)))
+
+    def done(self):
+        rc = self._lib.nlse_group_done(self._h)
+        if rc < 0:
+            _check(-rc)
+        return bool(rc)
+
+    def close(self):
+        if self._h:
+            self._lib.nlse_group_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -14,11 +14,9 @@ namespace nlse {
 // works on XCD b mod 8, which owns a contiguous eighth of the list, so the
 // tiles of one problem (and their shared halos) stay in one L2.
 template <typename T, int METHOD, int TI, int FIRST, int LAST>
-__global__ __launch_bounds__(256) void batch2d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
-                                                      c2<T>* __restrict__ acc, c2<T>* out,
-                                                      const T* __restrict__ V, const BatchProb2<T>* __restrict__ probs,
-                                                      const int* __restrict__ tile_prob, int ntiles,
-                                                      c2<T>* __restrict__ ulo)
+__global__ __launch_bounds__(256) void batch2d_kernel(const BatchProb2<T>* __restrict__ probs,
+                                                      const int* __restrict__ tile_prob, int ntiles, int r_yin,
+                                                      int r_u, int r_out)
 {
     using L = TileLds<T, METHOD, TI, FIRST, LAST>;
     __shared__ c2<T> sY[L::NY];
@@ -32,19 +30,16 @@ __global__ __launch_bounds__(256) void batch2d_kernel(const c2<T>* __restrict__ 
         const int lt = t - P.tile0;
         const int ti = lt / P.tiles_j, tj = lt - ti * P.tiles_j;
         fused2d_tile<T, METHOD, TI, FIRST, LAST>(sY, sD, ti * TI - P.ioff, tj * TJ2 - P.joff, P.N, P.M,
-                                                 yin + P.off, u + P.off, acc + P.off, out + P.off, V + P.off, P.c,
-                                                 P.BC, ulo ? ulo + P.off : nullptr);
+                                                 P.buf[r_yin], P.buf[r_u], P.acc, P.buf[r_out], P.V, P.c, P.BC,
+                                                 P.ulo);
     }
 }
 
 // the same walk over row-sweep tiles (tile width 64 - 2H per problem)
 template <typename T, int METHOD, int TI, int FIRST, int LAST, bool COMP, int NW>
-__global__ __launch_bounds__(NW * 64) void batch2d_rows_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
-                                                           c2<T>* __restrict__ acc, c2<T>* out,
-                                                           const T* __restrict__ V,
-                                                           const BatchProb2<T>* __restrict__ probs,
+__global__ __launch_bounds__(NW * 64) void batch2d_rows_kernel(const BatchProb2<T>* __restrict__ probs,
                                                            const int* __restrict__ tile_prob, int ntiles,
-                                                           c2<T>* __restrict__ ulo)
+                                                           int r_yin, int r_u, int r_out)
 {
     __shared__ RowsLds<T, METHOD, FIRST, LAST, TI> sm;
     constexpr int W = RowsCfg<METHOD, FIRST, LAST, TI>::W;
@@ -57,14 +52,14 @@ __global__ __launch_bounds__(NW * 64) void batch2d_rows_kernel(const c2<T>* __re
         const int lt = t - P.tile0;
         const int ti = lt / P.tiles_j, tj = lt - ti * P.tiles_j;
         const int i0 = ti * TI - P.ioff, j0 = tj * W - P.joff;
-        c2<T>* lo = COMP ? ulo + P.off : nullptr;
+        c2<T>* lo = COMP ? P.ulo : nullptr;
         if (rows_inner<METHOD, FIRST, LAST, TI>(i0, j0, P.N, P.M))
-            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true, NW>(sm, i0, j0, P.N, P.M, yin + P.off, u + P.off,
-                                                                    acc + P.off, out + P.off, V + P.off, P.c, P.BC, lo);
+            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, true, NW>(sm, i0, j0, P.N, P.M, P.buf[r_yin], P.buf[r_u],
+                                                                    P.acc, P.buf[r_out], P.V, P.c, P.BC, lo);
         else
-            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, false, NW>(sm, i0, j0, P.N, P.M, yin + P.off, u + P.off,
-                                                                     acc + P.off, out + P.off, V + P.off, P.c, P.BC,
-                                                                     lo);
+            rows2d_tile<T, METHOD, FIRST, LAST, TI, COMP, false, NW>(sm, i0, j0, P.N, P.M, P.buf[r_yin],
+                                                                     P.buf[r_u], P.acc, P.buf[r_out], P.V, P.c,
+                                                                     P.BC, lo);
         __syncthreads();  // the next tile reuses the LDS
     }
 }
@@ -73,12 +68,9 @@ __global__ __launch_bounds__(NW * 64) void batch2d_rows_kernel(const c2<T>* __re
 // of a workgroup takes its own unit (a segment of S rows x one strip of
 // 64 - 2H columns of one problem); waves never synchronise
 template <typename T, int METHOD, int FIRST, int LAST, bool COMP>
-__global__ __launch_bounds__(256) void batch2d_strip_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
-                                                            c2<T>* __restrict__ acc, c2<T>* out,
-                                                            const T* __restrict__ V,
-                                                            const BatchProb2<T>* __restrict__ probs,
+__global__ __launch_bounds__(256) void batch2d_strip_kernel(const BatchProb2<T>* __restrict__ probs,
                                                             const int* __restrict__ tile_prob, int ntiles, int S,
-                                                            c2<T>* __restrict__ ulo)
+                                                            int r_yin, int r_u, int r_out)
 {
     constexpr int W = StripCfg<METHOD, FIRST, LAST>::W;
     const int xcd = blockIdx.x & 7, nper = gridDim.x >> 3;
@@ -90,9 +82,9 @@ __global__ __launch_bounds__(256) void batch2d_strip_kernel(const c2<T>* __restr
         const BatchProb2<T>& P = probs[tile_prob[t]];
         const int lt = t - P.tile0;
         const int ti = lt / P.tiles_j, tj = lt - ti * P.tiles_j;
-        strip2d_wave<T, METHOD, FIRST, LAST, COMP>(ti * S - P.ioff, S, tj * W - P.joff, P.N, P.M, yin + P.off,
-                                                   u + P.off, acc + P.off, out + P.off, V + P.off, P.c, P.BC,
-                                                   COMP ? ulo + P.off : nullptr);
+        strip2d_wave<T, METHOD, FIRST, LAST, COMP>(ti * S - P.ioff, S, tj * W - P.joff, P.N, P.M, P.buf[r_yin],
+                                                   P.buf[r_u], P.acc, P.buf[r_out], P.V, P.c, P.BC,
+                                                   COMP ? P.ulo : nullptr);
     }
 }
 
@@ -100,14 +92,14 @@ template <typename T, int METHOD, int FIRST, int LAST>
 static hipError_t launch_b_strips(const BatchArgs<T>& a, hipStream_t st)
 {
     if constexpr (std::is_same<T, float>::value && LAST == 4) {
-        if (a.ulo) {
+        if (a.comp) {
             hipLaunchKernelGGL((batch2d_strip_kernel<T, METHOD, FIRST, LAST, true>), dim3(a.grid), dim3(256), 0, st,
-                               a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.tile, a.ulo);
+                               a.probs, a.tile_prob, a.ntiles, a.tile, a.r_yin, a.r_u, a.r_out);
             return hipGetLastError();
         }
     }
-    hipLaunchKernelGGL((batch2d_strip_kernel<T, METHOD, FIRST, LAST, false>), dim3(a.grid), dim3(256), 0, st, a.yin,
-                       a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.tile, a.ulo);
+    hipLaunchKernelGGL((batch2d_strip_kernel<T, METHOD, FIRST, LAST, false>), dim3(a.grid), dim3(256), 0, st,
+                       a.probs, a.tile_prob, a.ntiles, a.tile, a.r_yin, a.r_u, a.r_out);
     return hipGetLastError();
 }
 
@@ -116,23 +108,22 @@ static hipError_t launch_b_rows(const BatchArgs<T>& a, hipStream_t st)
 {
     constexpr int NW = rows_waves<METHOD, FIRST, LAST, TI>();
     if constexpr (std::is_same<T, float>::value && LAST == 4) {
-        if (a.ulo) {
+        if (a.comp) {
             hipLaunchKernelGGL((batch2d_rows_kernel<T, METHOD, TI, FIRST, LAST, true, NW>), dim3(a.grid),
-                               dim3(NW * 64), 0, st, a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles,
-                               a.ulo);
+                               dim3(NW * 64), 0, st, a.probs, a.tile_prob, a.ntiles, a.r_yin, a.r_u, a.r_out);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((batch2d_rows_kernel<T, METHOD, TI, FIRST, LAST, false, NW>), dim3(a.grid), dim3(NW * 64), 0,
-                       st, a.yin, a.u, a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.ulo);
+                       st, a.probs, a.tile_prob, a.ntiles, a.r_yin, a.r_u, a.r_out);
     return hipGetLastError();
 }
 
 template <typename T, int METHOD, int TI, int FIRST, int LAST>
 static hipError_t launch_b(const BatchArgs<T>& a, hipStream_t st)
 {
-    hipLaunchKernelGGL((batch2d_kernel<T, METHOD, TI, FIRST, LAST>), dim3(a.grid), dim3(256), 0, st, a.yin, a.u,
-                       a.acc, a.out, a.V, a.probs, a.tile_prob, a.ntiles, a.ulo);
+    hipLaunchKernelGGL((batch2d_kernel<T, METHOD, TI, FIRST, LAST>), dim3(a.grid), dim3(256), 0, st, a.probs,
+                       a.tile_prob, a.ntiles, a.r_yin, a.r_u, a.r_out);
     return hipGetLastError();
 }
 

@@ -18,7 +18,9 @@ struct nlse_batch {
     std::vector<double> s, a, h2, k;
     long ntot;
     void *U, *Ulo, *Y0, *Y1, *acc, *V;
-    void* ubase;
+    void* ubase;     // the allocations U and Y1 started in (they swap after fused steps):
+    void* ybase;     // the problem table's buffers 0 and 2
+    double* ws;      // diagnostics: 3 results per problem, then 3 * 1024 block partials
     void* probs;     // device BatchProb2<T>[nprob]
     int* tile_prob;  // device: problem of each tile
     int ntiles;
@@ -86,7 +88,13 @@ int build_tiles_t(nlse_batch* b)
         q.tiles_i = (q.N + q.ioff + TI - 1) / TI;
         q.tiles_j = (q.M + q.joff + TW - 1) / TW;
         q.tile0 = t0;
-        q.off = b->off[p];
+        const long o = b->off[p];
+        q.buf[0] = (c2<T>*)b->ubase + o;
+        q.buf[1] = (c2<T>*)b->Y0 + o;
+        q.buf[2] = (c2<T>*)b->ybase + o;
+        q.acc = (c2<T>*)b->acc + o;
+        q.V = (const T*)b->V + o;
+        q.ulo = b->Ulo ? (c2<T>*)b->Ulo + o : nullptr;
         coefs2d(q.c, (T)b->s[p], (T)b->a[p], (T)b->h2[p], (T)b->k[p]);
         const int nt = q.tiles_i * q.tiles_j;
         tp.insert(tp.end(), nt, p);
@@ -110,19 +118,20 @@ int build_tiles_t(nlse_batch* b)
 
 int build_tiles(nlse_batch* b) { return dbl(b) ? build_tiles_t<double>(b) : build_tiles_t<float>(b); }
 
+// the role (table buffer) of one of the batch's state buffers
+int role(const nlse_batch* b, const void* p) { return p == b->ubase ? 0 : (p == b->Y0 ? 1 : 2); }
+
 template <typename T>
 int launch_t(nlse_batch* b, int first, int last, const void* yin, void* out)
 {
     BatchArgs<T> a;
-    a.yin = (const c2<T>*)yin;
-    a.u = (const c2<T>*)b->U;
-    a.acc = (c2<T>*)b->acc;
-    a.out = (c2<T>*)out;
-    a.V = (const T*)b->V;
+    a.r_yin = role(b, yin);
+    a.r_u = role(b, b->U);
+    a.r_out = role(b, out);
+    a.comp = b->Ulo != nullptr;
     a.probs = (const BatchProb2<T>*)b->probs;
     a.tile_prob = b->tile_prob;
     a.ntiles = b->ntiles;
-    a.ulo = (c2<T>*)b->Ulo;
     a.method = b->method;
     a.first = first;
     a.last = last;
@@ -258,6 +267,11 @@ int nlse_batch_create(nlse_batch** out, int nprob, const int* N, const int* M, c
         return fail(NLSE_ENOMEM, "hipMalloc failed");
     }
     b->ubase = b->U;
+    b->ybase = b->Y1;
+    if (dev_malloc((void**)&b->ws, sizeof(double) * (3 * 1024 + 3 * (size_t)nprob)) != hipSuccess) {
+        nlse_batch_destroy(b);
+        return fail(NLSE_ENOMEM, "hipMalloc failed");
+    }
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
         nlse_batch_destroy(b);
@@ -287,7 +301,7 @@ void nlse_batch_destroy(nlse_batch* b)
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graphs(b);
-    void* bufs[8] = {b->U, b->Ulo, b->Y0, b->Y1, b->acc, b->V, b->probs, b->tile_prob};
+    void* bufs[9] = {b->U, b->Ulo, b->Y0, b->Y1, b->acc, b->V, b->probs, b->tile_prob, b->ws};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -360,8 +374,7 @@ int nlse_batch_step_timed(nlse_batch* b, int steps, double* ms)
 int nlse_batch_diagnostics(nlse_batch* b, double* out)
 {
     if (!b || !out) return fail(NLSE_EINVAL, "null argument");
-    double* d = nullptr;
-    HIPCHK(dev_malloc((void**)&d, sizeof(double) * (3 * 1024 + 3 * (size_t)b->nprob)));
+    double* d = b->ws;
     double* part = d + 3 * (size_t)b->nprob;
     const size_t eb = b->prec == 2 ? 16 : 8;
     hipError_t e = hipSuccess;
@@ -374,7 +387,6 @@ int nlse_batch_diagnostics(nlse_batch* b, double* out)
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, d, 3 * sizeof(double) * b->nprob, hipMemcpyDeviceToHost, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-    (void)hipFree(d);
     if (e != hipSuccess) return fail(NLSE_EHIP, std::string("diagnostics: ") + hipGetErrorString(e));
     return NLSE_OK;
 }

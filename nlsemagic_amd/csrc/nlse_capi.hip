@@ -1108,3 +1108,183 @@ int NLSE3D_VORTICITY_HIP(const double* Ur, const double* Ui, double lh2, double 
 }
 
 }  // extern "C"
+
+// ---- groups of resident 2D solvers advanced by shared launches
+//
+// The solvers keep their own buffers; a problem table points the batched 2D
+// kernels (nlse2d_batch.hip) at them, so one persistent launch per fused pair
+// of RK4 stages walks the tiles of every member.  The tile code and each
+// member's coefficients are the solver's own: a member's result is bit for
+// bit the result of stepping it alone.
+
+struct nlse_group {
+    std::vector<nlse_solver*> mem;
+    int method, prec;
+    void* probs;     // device BatchProb2<T>[n]
+    int* tile_prob;  // device: member of each tile
+    int ntiles, grid;
+    int parity;      // fused steps taken mod 2: buffers 0 and 2 of the table have swapped roles
+    hipStream_t stream;
+};
+
+namespace {
+
+constexpr int GROUP_ROWS = 8;  // row-sweep tiles of 8 rows, two stages per launch
+
+template <typename T>
+int group_table_t(nlse_group* g)
+{
+    const int R = g->method == 2 ? 2 : 1, TW = 64 - 4 * R;
+    std::vector<BatchProb2<T>> P(g->mem.size());
+    std::vector<int> tp;
+    int t0 = 0;
+    for (size_t p = 0; p < g->mem.size(); ++p) {
+        const nlse_solver* h = g->mem[p];
+        BatchProb2<T>& q = P[p];
+        q.N = h->N;
+        q.M = h->M;
+        q.BC = h->BC;
+        q.ioff = (q.N % GROUP_ROWS == 1) ? 1 : 0;
+        q.joff = (q.M % TW == 1) ? 1 : 0;
+        q.tiles_i = (q.N + q.ioff + GROUP_ROWS - 1) / GROUP_ROWS;
+        q.tiles_j = (q.M + q.joff + TW - 1) / TW;
+        q.tile0 = t0;
+        q.buf[0] = (c2<T>*)h->U;  // the state when the group forms
+        q.buf[1] = (c2<T>*)h->Y0;
+        q.buf[2] = (c2<T>*)h->Y1;
+        q.acc = (c2<T>*)h->acc;
+        q.V = (const T*)h->V;
+        q.ulo = (c2<T>*)h->Ulo;
+        if constexpr (std::is_same<T, double>::value) q.c = h->cd;
+        else q.c = h->cf;
+        const int nt = q.tiles_i * q.tiles_j;
+        tp.insert(tp.end(), nt, (int)p);
+        t0 += nt;
+    }
+    if (dev_malloc(&g->probs, sizeof(BatchProb2<T>) * P.size()) != hipSuccess ||
+        dev_malloc((void**)&g->tile_prob, sizeof(int) * tp.size()) != hipSuccess)
+        return fail(NLSE_ENOMEM, "hipMalloc failed");
+    HIPCHK(hipMemcpyAsync(g->probs, P.data(), sizeof(BatchProb2<T>) * P.size(), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipMemcpyAsync(g->tile_prob, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice, g->stream));
+    HIPCHK(hipStreamSynchronize(g->stream));
+    g->ntiles = t0;
+    return NLSE_OK;
+}
+
+template <typename T>
+int group_launch_t(nlse_group* g, int first, int last, int r_yin, int r_out)
+{
+    BatchArgs<T> a;
+    a.probs = (const BatchProb2<T>*)g->probs;
+    a.tile_prob = g->tile_prob;
+    a.ntiles = g->ntiles;
+    a.r_yin = r_yin;
+    a.r_u = g->parity ? 2 : 0;
+    a.r_out = r_out;
+    a.comp = g->prec == 3;
+    a.method = g->method;
+    a.first = first;
+    a.last = last;
+    a.tile = GROUP_ROWS;
+    a.grid = g->grid;
+    a.kernel = KERNEL_ROWS;
+    const hipError_t e = launch_batch2d<T>(a, g->stream);
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("group launch: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
+
+// one RK4 step of every member: stages 1-2 into buffer 1 (Y_2, ktot), 3-4
+// into the buffer not holding U, which then holds U
+int group_step1(nlse_group* g)
+{
+    const int cur = g->parity ? 2 : 0, other = 2 - cur;
+    const bool d = g->prec == 2;
+    int rc = d ? group_launch_t<double>(g, 1, 2, cur, 1) : group_launch_t<float>(g, 1, 2, cur, 1);
+    if (rc == NLSE_OK) rc = d ? group_launch_t<double>(g, 3, 4, 1, other) : group_launch_t<float>(g, 3, 4, 1, other);
+    if (rc == NLSE_OK) g->parity ^= 1;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nlse_group_create(nlse_group** out, nlse_solver* const* members, int n)
+{
+    if (!out) return fail(NLSE_EINVAL, "null group pointer");
+    *out = nullptr;
+    if (!members || n < 1) return fail(NLSE_EINVAL, "a group needs at least one member");
+    for (int i = 0; i < n; ++i) {
+        const nlse_solver* h = members[i];
+        if (!h) return fail(NLSE_EINVAL, "null member");
+        if (h->dim != 2) return fail(NLSE_EINVAL, "group members must be 2D solvers");
+        if (h->method != members[0]->method || h->prec != members[0]->prec)
+            return fail(NLSE_EINVAL, "group members must share the method and the precision");
+        for (int j = 0; j < i; ++j)
+            if (members[j] == h) return fail(NLSE_EINVAL, "a solver appears twice in the group");
+    }
+    nlse_group* g = new nlse_group();
+    g->mem.assign(members, members + n);
+    g->method = members[0]->method;
+    g->prec = members[0]->prec;
+    g->probs = nullptr;
+    g->tile_prob = nullptr;
+    g->parity = 0;
+    g->stream = nullptr;
+    int cus = 256;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0);
+    g->grid = 8 * ((cus * 4 + 7) / 8);
+    // the members' queued work finishes before the group touches their buffers
+    for (nlse_solver* h : g->mem)
+        if (hipStreamSynchronize(h->stream) != hipSuccess) {
+            delete g;
+            return fail(NLSE_EHIP, "member stream synchronisation failed");
+        }
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return fail(NLSE_EHIP, "stream creation failed");
+    }
+    const int rc = g->prec == 2 ? group_table_t<double>(g) : group_table_t<float>(g);
+    if (rc != NLSE_OK) {
+        nlse_group_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return NLSE_OK;
+}
+
+int nlse_group_step(nlse_group* g, int steps)
+{
+    if (!g) return fail(NLSE_EINVAL, "null group");
+    if (steps < 0) return fail(NLSE_EINVAL, "steps must be >= 0");
+    for (int i = 0; i < steps; ++i) {
+        const int rc = group_step1(g);
+        if (rc != NLSE_OK) return rc;
+    }
+    return NLSE_OK;
+}
+
+int nlse_group_done(nlse_group* g)
+{
+    if (!g) return fail(NLSE_EINVAL, "null group");
+    const hipError_t e = hipStreamQuery(g->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return -fail(NLSE_EHIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+void nlse_group_destroy(nlse_group* g)
+{
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    // an odd number of steps leaves each member's U in its Y1 buffer: the
+    // solvers swap, as their own fused steps do
+    if (g->parity)
+        for (nlse_solver* h : g->mem) std::swap(h->U, h->Y1);
+    if (g->probs) (void)hipFree(g->probs);
+    if (g->tile_prob) (void)hipFree(g->tile_prob);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+}  // extern "C"

@@ -43,29 +43,33 @@ template <typename T> hipError_t launch_step2d(const StageArgs<T>& a, hipStream_
 // 3D, stages 1-2 (first) or 3-4 in one plane-marching launch (CD or 2SHOC); not MSD
 template <typename T> hipError_t launch_pair3d(const StageArgs<T>& a, bool first, hipStream_t st);
 
-// tiles of problem p are [tile0, tile0 + tiles_i*tiles_j); fields of every
-// per-point array of problem p start at element `off`
+// tiles of problem p are [tile0, tile0 + tiles_i*tiles_j).  Its fields:
+// buf[0..2] the three buffers the state and the stage values rotate through
+// (a launch names the roles: stage input, U, output), acc (ktot), V and ulo
+// (the low part of a compensated state, else null).  The problems of a batch
+// live in shared buffers; a group of resident solvers keeps each problem in
+// its own solver's buffers.
 template <typename T>
 struct BatchProb2 {
     int N, M, BC;
     int tiles_i, tiles_j, ioff, joff, tile0;
-    long off;
+    c2<T>* buf[3];
+    c2<T>* acc;
+    const T* V;
+    c2<T>* ulo;
     Coefs<T> c;
 };
 
 // One persistent launch of RK4 stages first..last over every tile of a batch
 // of 2D problems (`tile` = rows per tile, or per strip segment for
-// KERNEL_STRIPS; `grid` workgroups, a multiple of 8).
+// KERNEL_STRIPS; `grid` workgroups, a multiple of 8): stage input
+// buf[r_yin], U = buf[r_u], output buf[r_out] of every problem.
 template <typename T> struct BatchArgs {
-    const c2<T>* yin;
-    const c2<T>* u;
-    c2<T>* acc;
-    c2<T>* out;
-    const T* V;
     const BatchProb2<T>* probs;
     const int* tile_prob;
     int ntiles;
-    c2<T>* ulo;
+    int r_yin, r_u, r_out;
+    bool comp;  // compensated single precision (every problem has ulo)
     int method, first, last, tile, grid;
     int kernel;  // KERNEL_TILES, KERNEL_ROWS or KERNEL_STRIPS
 };

@@ -14,6 +14,13 @@ compensated FP32, step count, initial state and potential).  The server:
   as its previous one completes (non-blocking stream queries), so requests
   that arrive while others run join the running mix without waiting for it
   to drain;
+- batches the 2D requests: the active 2D requests of one method and
+  precision take their chunks together as a group (nlse_group_*), one
+  persistent launch per pair of RK4 stages over the tiles of all of them
+  (each in its own buffers), the chunk being the smallest number of steps
+  any of them has left (at most ``chunk_steps``); a 2D request that is ready
+  while others of its kind are still working waits for them, so a request
+  that arrives joins the next chunk of its kind's group;
 - releases a finished request's memory at once and records its latency
   (finish - arrival).
 
@@ -32,7 +39,7 @@ from typing import Callable, List, Optional
 
 import numpy as np
 
-from . import Solver, footprint_bytes
+from . import Group, Solver, footprint_bytes
 
 
 @dataclasses.dataclass
@@ -93,13 +100,16 @@ class Request:
 
 class Server:
     def __init__(self, budget_bytes: float = 200e9, max_active: int = 16, chunk_steps: int = 200,
-                 keep_results: bool = True):
+                 keep_results: bool = True, group_2d: bool = True):
         self.budget = float(budget_bytes)
         self.max_active = int(max_active)
         self.chunk = int(chunk_steps)
         self.keep_results = keep_results
+        self.group_2d = bool(group_2d)
         self.peak_bytes = 0
         self.peak_active = 0
+        self.groups_formed = 0
+        self.grouped_request_chunks = 0
 
     def _admit(self, r: Request, now: float):
         sol = Solver(r.shape, r.method, r.precision, r.BC, r.s, r.a, r.h2, r.k)
@@ -112,10 +122,13 @@ class Server:
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         waiting: List[Request] = []
         active = {}  # rid -> [request, solver, steps_left, footprint]
+        groups: List[Group] = []  # running groups of 2D requests
+        grouped = {}              # id(group) -> its requests
+        in_group = set()          # requests whose solver a running group holds
         used = 0
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0
-        while pending or waiting or active:
+        while pending or waiting or active or groups:
             now = clock()
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
@@ -132,9 +145,20 @@ class Server:
             self.peak_bytes = max(self.peak_bytes, used)
             self.peak_active = max(self.peak_active, len(active))
             progressed = False
+            # running groups: a finished one hands its members back
+            for g in [g for g in groups if g.done()]:
+                g.close()
+                groups.remove(g)
+                for rid in grouped.pop(id(g)):
+                    in_group.discard(rid)
+                progressed = True
+            ready = {}  # (method, precision) -> 2D requests ready for a chunk
+            busy = set()  # kinds with a 2D request still working on a chunk
             for rid in list(active):
                 r, sol, left, fp = active[rid]
-                if not sol.done():
+                if rid in in_group or not sol.done():
+                    if self.group_2d and len(r.shape) == 2:
+                        busy.add((r.method, r.precision))
                     continue
                 progressed = True
                 if left == 0:
@@ -147,16 +171,39 @@ class Server:
                     used -= fp
                     del active[rid]
                     continue
+                if self.group_2d and len(r.shape) == 2:
+                    ready.setdefault((r.method, r.precision), []).append(rid)
+                    continue
                 n = min(self.chunk, left)
                 sol.step(n)
                 active[rid][2] = left - n
+            for kind, rids in ready.items():
+                if kind in busy:
+                    continue  # wait for the busy ones, then take the next chunk together
+                if len(rids) == 1:
+                    r, sol, left, fp = active[rids[0]]
+                    n = min(self.chunk, left)
+                    sol.step(n)
+                    active[rids[0]][2] = left - n
+                    continue
+                n = min([self.chunk] + [active[rid][2] for rid in rids])
+                g = Group([active[rid][1] for rid in rids])
+                g.step(n)
+                groups.append(g)
+                grouped[id(g)] = rids
+                for rid in rids:
+                    in_group.add(rid)
+                    active[rid][2] -= n
+                self.groups_formed += 1
+                self.grouped_request_chunks += len(rids)
             if not progressed:
                 if not active and pending and not waiting:
                     time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
                 else:
                     time.sleep(50e-6)
         makespan = clock()
-        return summarize(requests, makespan, peak_bytes=self.peak_bytes, peak_active=self.peak_active)
+        return summarize(requests, makespan, peak_bytes=self.peak_bytes, peak_active=self.peak_active,
+                         groups_formed=self.groups_formed, grouped_request_chunks=self.grouped_request_chunks)
 
 
 def run_one_at_a_time(requests: List[Request], precision: Optional[int] = 2) -> dict:
