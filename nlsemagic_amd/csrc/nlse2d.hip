@@ -94,8 +94,11 @@ static hipError_t launch_strip(const StageArgs<T>& a, hipStream_t st)
 }
 
 // Row-sweep LDS tiles (nlse2d_rows.h): one workgroup per tile of TI rows.
+// At most 128 VGPRs (four waves per SIMD): the FP64 stage-3-4 tile would take
+// 132 and run three waves per SIMD, slower than with four waves and a few
+// registers spilled.
 template <typename T, int METHOD, int FIRST, int LAST, int TI, bool COMP, int NW>
-__global__ __launch_bounds__(NW * 64) void rows2d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) void rows2d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
                                                      c2<T>* __restrict__ acc, c2<T>* out, const T* __restrict__ V,
                                                      Coefs<T> c, Grid2 g, int BC, c2<T>* __restrict__ ulo)
 {

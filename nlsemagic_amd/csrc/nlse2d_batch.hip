@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void batch2d_kernel(const BatchProb2<T>* __res
 
 // the same walk over row-sweep tiles (tile width 64 - 2H per problem)
 template <typename T, int METHOD, int TI, int FIRST, int LAST, bool COMP, int NW>
-__global__ __launch_bounds__(NW * 64) void batch2d_rows_kernel(const BatchProb2<T>* __restrict__ probs,
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(4))) void batch2d_rows_kernel(const BatchProb2<T>* __restrict__ probs,
                                                            const int* __restrict__ tile_prob, int ntiles,
                                                            int r_yin, int r_u, int r_out)
 {

@@ -140,24 +140,31 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
     c2<T> Yc[RW], Uc[RW], Ac[RW], Lc[RW];
     T Vc[RW];
 
-    // phase 0: every load of the tile in flight at once
+    // phase 0: every load of the tile in flight at once, the stage input
+    // first (the D phase needs nothing else, so it can start while V, U and
+    // ktot are still arriving); no branches around the loads: a row outside
+    // the range a field is used in reads a row inside it (out-of-grid rows
+    // read clamped rows: values no owned point uses)
+    auto rowp = [&](int e) -> long {
+        const int g = g0 + e;
+        const int gc = INNER ? g : (g < 0 ? 0 : (g >= N ? N - 1 : g));
+        return (long)gc * M + jc;
+    };
+#pragma unroll
+    for (int k = 0; k < RW; ++k) Yc[k] = yin[rowp(min(NW * k + w, TE - 1))];
+#pragma unroll
+    for (int k = 0; k < RW; ++k) Vc[k] = V[rowp(min(NW * k + w, TE - 1))];
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
         const int e = NW * k + w;
-        const int g = g0 + e;
-        const int gc = INNER ? g : (g < 0 ? 0 : (g >= N ? N - 1 : g));
-        const long p = (long)gc * M + jc;
-        const bool row = e < TE;  // every extended row is read (out-of-grid rows give garbage, never used)
-        const bool frow = e >= R && e < TE - R;  // rows the first fused stage updates
-        Yc[k] = row ? yin[p] : z;
-        Vc[k] = row ? V[p] : (T)0;
-        if constexpr (FIRST > 1) {
-            Uc[k] = frow ? u[p] : z;
-            Ac[k] = frow ? acc[p] : z;
+        if constexpr (FIRST > 1) {  // the rows the first fused stage updates: R .. TE-R-1
+            const long pf = rowp(e < R ? R : (e >= TE - R ? TE - R - 1 : e));
+            Uc[k] = u[pf];
+            Ac[k] = acc[pf];
         } else {
             Ac[k] = z;
         }
-        if constexpr (COMP && LAST == 4) Lc[k] = (e >= H && e < H + TI) ? ulo[p] : z;
+        if constexpr (COMP && LAST == 4) Lc[k] = ulo[rowp(e < H ? H : (e >= H + TI ? H + TI - 1 : e))];
     }
 #pragma unroll
     for (int k = 0; k < RW; ++k) {
@@ -172,13 +179,6 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
         }
     }
     __syncthreads();
-    // the D column pass: lane l of wave 0 takes extended row l & 31 on side
-    // l >> 5 (0: column 0 at lane lL, partner column 1; 1: column M-1 at lane
-    // lR, partner column M-2) of the interior grid rows (rows 0 and N-1,
-    // corners included, belong to the row pass)
-    const int cp_e = lane & 31, cp_side = lane >> 5;
-    const int cp_lb = cp_side ? lR : lL, cp_lq = cp_side ? lR - 1 : lL + 1;
-    const bool cp_on = w == 0 && cp_lb >= 0 && cp_lb < 64 && cp_e < TE && interior(g0 + cp_e);
 
 #pragma unroll
     for (int q = 0; q < NF; ++q) {
@@ -216,13 +216,17 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                 ROWS_SCHED_FENCE();
             }
             __syncthreads();
-            if (colmsd) {  // MSD columns 0 / M-1 of the interior rows from columns 1 / M-2
-                if (cp_on && cp_e >= dlo && cp_e < dhi) {
-                    const int e = cp_e;
-                    sm.d[1 + (e - 1) * 64 + cp_lb] =
-                        msd_d<T>(sd(e, cp_lq - lane), sy(e, cp_lq - lane), sm.bv[e][cp_side ? 2 : 1], sy(e, cp_lb - lane),
-                                 sm.bv[e][cp_side ? 3 : 0], c);
-                }
+            if (colmsd) {
+                // the D column pass: lane l of wave 0 takes extended row l & 31
+                // on side l >> 5 (0: column 0 at lane lL, partner column 1;
+                // 1: column M-1 at lane lR, partner column M-2) of the interior
+                // grid rows (rows 0 and N-1, corners included, belong to the
+                // row pass)
+                const int e = lane & 31, side = lane >> 5;
+                const int lb = side ? lR : lL, lq = side ? lR - 1 : lL + 1;
+                if (w == 0 && lb >= 0 && lb < 64 && e < TE && e >= dlo && e < dhi && interior(g0 + e))
+                    sm.d[1 + (e - 1) * 64 + lb] = msd_d<T>(sd(e, lq - lane), sy(e, lq - lane), sm.bv[e][side ? 2 : 1],
+                                                           sy(e, lb - lane), sm.bv[e][side ? 3 : 0], c);
                 __syncthreads();
             }
             if (top || bot) {  // MSD rows 0 / N-1 from rows 1 / N-2 (uniform over the workgroup)
