@@ -178,7 +178,8 @@ __global__ __launch_bounds__(NW * 64) void cdpair3d_kernel(const c2<T>* __restri
     cdpair_tile<T, FIRST, TJ, NW>(sm, ci * g.IC, jt * TJ, kt * 60, g, yin, u, acc, yout, V, c, BC, ulo);
 }
 
-// 2SHOC, two stages per launch on plane-marching tiles (one extended row per wave)
+// 2SHOC, two stages per launch on plane-marching tiles (one extended row per
+// wave; D and the running ktot in LDS)
 template <typename T, int FIRST, int TJ, int NW>
 __global__ __launch_bounds__(NW * 64) void shocpair3d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
                                                              c2<T>* __restrict__ acc, c2<T>* __restrict__ yout,
@@ -193,29 +194,13 @@ __global__ __launch_bounds__(NW * 64) void shocpair3d_kernel(const c2<T>* __rest
     shocpair_tile<T, FIRST, TJ, NW>(sm, ci * g.IC, jt * TJ, kt * 56, g, yin, u, acc, yout, V, c, BC, ulo);
 }
 
-// the same pair with D in LDS and the running ktot in memory (FP64)
-template <typename T, int FIRST, int TJ, int NW>
-__global__ __launch_bounds__(NW * 64) void shoclean3d_kernel(const c2<T>* __restrict__ yin, const c2<T>* u,
-                                                             c2<T>* __restrict__ acc, c2<T>* __restrict__ yout,
-                                                             const T* __restrict__ V, Coefs<T> c, March3 g, int BC,
-                                                             c2<T>* ulo)
-{
-    __shared__ ShocLeanLds<T, TJ> sm;
-    const int t = (blockIdx.x & 7) * g.per_xcd + (blockIdx.x >> 3);
-    if (t >= g.tj * g.tk * g.ti) return;
-    const int kt = t % g.tk, rest = t / g.tk;
-    const int jt = rest % g.tj, ci = rest / g.tj;
-    shoclean_tile<T, FIRST, TJ, NW>(sm, ci * g.IC, jt * TJ, kt * 56, g, yin, u, acc, yout, V, c, BC, ulo);
-}
-
 template <typename T>
 static hipError_t launch_shocpair3d(const StageArgs<T>& a, bool first, hipStream_t st)
 {
-    // FP64: 4 owned rows on 12 waves (one extended row per wave), D and the
-    // running ktot in LDS (160 KiB, one workgroup per CU); FP32: 8 rows on 16
-    // waves, D of three planes in registers (82 KB)
-    constexpr bool LEAN = std::is_same<T, double>::value;
-    constexpr int TJ = LEAN ? 4 : 8, NW = TJ + 8;
+    // FP64: 4 owned rows on 12 waves (LDS rings of 160 KiB, one workgroup
+    // per CU); FP32: 8 rows on 16 waves (120 KiB)
+    constexpr int TJ = std::is_same<T, double>::value ? 4 : 8, NW = TJ + 8;
+    if ((long)a.N * a.M > 0x7fffffffL) return hipErrorInvalidValue;  // 32-bit offsets within a plane
     March3 g;
     g.N = a.N;
     g.M = a.M;
@@ -225,18 +210,6 @@ static hipError_t launch_shocpair3d(const StageArgs<T>& a, bool first, hipStream
     g.tk = (a.M + 55) / 56;
     g.ti = (a.L + g.IC - 1) / g.IC;
     g.per_xcd = (g.tj * g.tk * g.ti + 7) / 8;
-    // (the lean tile keeps 32-bit offsets within a plane)
-    if constexpr (LEAN) {
-        if ((long)a.N * a.M <= 0x7fffffffL) {
-            if (first)
-                hipLaunchKernelGGL((shoclean3d_kernel<T, 1, TJ, NW>), dim3(8 * g.per_xcd), dim3(NW * 64), 0, st,
-                                   a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
-            else
-                hipLaunchKernelGGL((shoclean3d_kernel<T, 3, TJ, NW>), dim3(8 * g.per_xcd), dim3(NW * 64), 0, st,
-                                   a.yin, a.u, a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);
-            return hipGetLastError();
-        }
-    }
     if (first)
         hipLaunchKernelGGL((shocpair3d_kernel<T, 1, TJ, NW>), dim3(8 * g.per_xcd), dim3(NW * 64), 0, st, a.yin, a.u,
                            a.acc, a.yout, a.V, a.c, g, a.BC, a.ulo);

@@ -57,14 +57,23 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
     const long si = (long)N * M;
     const c2<T> z = mk<T>(0, 0);
 
+    constexpr bool HAS_LO = SECOND && std::is_same<T, float>::value;
+    // row offsets (j * M + k, clamped into the grid) of each extended row and
+    // of the owned row whose U_lo it loads (rows outside 2 .. TE-3 load a copy
+    // of the nearest owned row, so every wave issues the same loads)
     int jrow[RW];
-    long rowoff[RW];
+    long rowoff[RW], rob[RW];
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
         const int e = NW * r + w;
         const int j = j0 - 2 + e;
         jrow[r] = j;
-        rowoff[r] = (long)(j < 0 ? 0 : (j >= N ? N - 1 : j)) * M + kc;
+        auto off = [&](int ee) -> long {
+            const int jj = j0 - 2 + ee;
+            return (long)(jj < 0 ? 0 : (jj >= N ? N - 1 : jj)) * M + kc;
+        };
+        rowoff[r] = off(e);
+        rob[r] = off(e < 2 ? 2 : (e > TE - 3 ? TE - 3 : e));
     }
     // register rings (slot = (plane - pstart) mod 3): the input Y and V of
     // planes p, p-1, p-2; the first stage's output and its ktot part of
@@ -72,7 +81,7 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
     // p, p-1, p-2
     c2<T> Yr[3][RW], Y1r[3][RW], Kr[3][RW], Br[SECOND ? 3 : 1][RW], Ar[SECOND ? 3 : 1][RW];
     T Vr[3][RW];
-    c2<T> Yn[RW], Bn[RW], An[RW];
+    c2<T> Yn[RW], Bn[RW], An[RW], Ln[RW];
     T Vn[RW];
 #pragma unroll
     for (int s = 0; s < 3; ++s)
@@ -87,9 +96,15 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
 
     const int pstart = i0 - 2;
     const int iend = min(i0 + g.IC, L);
-    const int nstep = (iend - i0) + 4;
+    const int nstep = (iend - i0 + 4 + 2) / 3 * 3;  // whole groups of three steps
+    // Every global load is issued on every step in the same order (clamped
+    // planes and rows; values of planes and rows a step does not use are never
+    // read), and the loads before the loop are those of a step, so the
+    // compiler waits for each load by count instead of draining all of them
+    // at each step.
+    auto plane = [&](int pl, int lo, int hi) -> long { return (long)(pl < lo ? lo : (pl > hi ? hi : pl)) * si; };
     auto load_plane = [&](int pl) {
-        const long pb = (long)(pl < 0 ? 0 : (pl >= L ? L - 1 : pl)) * si;
+        const long pb = plane(pl, 0, L - 1);
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
             Yn[r] = yin[pb + rowoff[r]];
@@ -110,12 +125,22 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
         }
         return (BC == BC_LAP0) ? lap0_f<T>(m, v, c.s) : z;
     };
+    // U_lo of the next step's update plane p-1 (compensated stages 3-4; read
+    // from U and unused when there is no low part)
+    auto load_lo = [&](int p) {
+        if constexpr (HAS_LO) {
+            const c2<T>* const src = ulo ? ulo : u;
+            const long pb = plane(p - 1, i0, iend - 1);
+#pragma unroll
+            for (int r = 0; r < RW; ++r) Ln[r] = src[pb + rob[r]];
+        }
+    };
     load_plane(pstart);
+    load_lo(pstart - 1);
     for (int q0 = 0; q0 < nstep; q0 += 3) {
         static_for<0, 3>([&](auto uc) {
             constexpr int u3 = decltype(uc)::value;
             const int st = q0 + u3;
-            if (st >= nstep) return;
             const int p = pstart + st;
             constexpr int a0 = u3, a1 = (u3 + 2) % 3, a2 = (u3 + 1) % 3;  // planes p, p-1, p-2 (p-3 -> a0)
             const int lnow = st & 1, lprev = lnow ^ 1;
@@ -131,10 +156,10 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
                 }
                 if (e < TE) sm.y[lnow][e][lane] = Yn[r];
             }
-            if (st + 1 < nstep) load_plane(p + 1);
-            // (b) the first stage at plane q = p-1, extended rows 1 .. TE-2
+            load_plane(p + 1);
+            // (b) the first stage at plane q = p-1 (i0-1 .. iend), extended rows 1 .. TE-2
             const int q = p - 1;
-            if (q >= 0 && q < L && st >= 1) {
+            if (q >= 0 && q < L && q >= i0 - 1 && q <= iend) {
 #pragma unroll
                 for (int r = 0; r < RW; ++r) {
                     const int e = NW * r + w;
@@ -194,8 +219,8 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
                         c2<T> b = Br[a2][r];
                         a.x = a.x + (T)1.0 * f.x;
                         a.y = a.y + (T)1.0 * f.y;
-                        if (std::is_same<T, float>::value && ulo) {
-                            c2<T> lo = ulo[pp];
+                        if (HAS_LO && ulo) {
+                            c2<T> lo = Ln[r];
                             comp_add<T>(b.x, lo.x, c.k6 * a.x);
                             comp_add<T>(b.y, lo.y, c.k6 * a.y);
                             ulo[pp] = lo;
@@ -206,6 +231,7 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
                     }
                 }
             }
+            load_lo(p);
             __syncthreads();  // this step's LDS planes are complete; the previous step's may be overwritten
         });
     }

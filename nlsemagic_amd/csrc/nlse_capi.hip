@@ -116,11 +116,12 @@ void drop_graphs(nlse_solver* h)
 }
 
 // RK4 stages per launch actually used: 2D 1, 2 or 4; 3D pairs of stages
-// (not MSD, whose partner may lie in another plane); 1 otherwise
+// (not MSD, whose partner may lie in another plane, nor planes of 2^31 points
+// or more, beyond the pair kernels' 32-bit offsets); 1 otherwise
 int fusion(const nlse_solver* h)
 {
     if (h->dim == 2) return h->fuse;
-    if (h->dim == 3 && h->BC != BC_MSD && h->fuse == 2) return 2;
+    if (h->dim == 3 && h->BC != BC_MSD && h->fuse == 2 && (long)h->N * h->M <= 0x7fffffffL) return 2;
     return 1;
 }
 
@@ -507,10 +508,8 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
         return fail(NLSE_EHIP, "stream/event creation failed");
     }
     h->ubase = h->U;
-    // 2D: pairs of stages; 3D: pairs (where the BC allows) except 2SHOC in
-    // double precision, whose pair kernel runs at half the one-stage rate
-    // (12 waves per workgroup, registers spilled)
-    h->fuse = (dim == 1 || (dim == 3 && method == 2 && precision == 2)) ? 1 : 2;
+    // 2D: pairs of stages; 3D: pairs where the BC allows
+    h->fuse = (dim == 1) ? 1 : 2;
     h->kern = KERNEL_ROWS;         // the fastest family on a 1024^2 grid and for 3D 2SHOC
     // 3D: plane-march rows; 1D: steps per launch (the fastest on N = 2048)
     h->tile = (dim == 3) ? 4 : (dim == 1 ? (method == 2 ? 16 : 32) : 8);
