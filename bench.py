@@ -631,8 +631,11 @@ def run_3d_counters(args):
 
 
 def run_serving(args):
-    """Poisson request stream (seed 2) through the continuous-batching server,
-    then the same requests one at a time through the plain FP64 path."""
+    """Poisson request streams (seed 2) at each --qps rate through the
+    continuous-batching server, each compared with the same requests one at
+    a time through the plain FP64 path.  The mix draws Dirichlet or MSD
+    boundaries and, for half the requests, a harmonic trap (request_mix
+    variety) unless --plain-mix is given."""
     import nlsemagic_amd as nm
     from nlsemagic_amd import serving as S
     warm = S.request_mix(12, qps=1000.0, seed=99)  # loads every kernel variant before timing
@@ -641,30 +644,44 @@ def run_serving(args):
         r.shape = tuple(min(n, 64) for n in r.shape)
         r.init = S.default_ic(r.shape, 0.1)
     S.Server(max_active=8, chunk_steps=10).run(warm)
-    reqs = S.request_mix(args.requests, qps=args.qps, seed=2)
-    srv = S.Server(budget_bytes=200e9, max_active=16, chunk_steps=200)
-    stats = srv.run(reqs)
-    served = {r.rid: r.result for r in reqs}
-    lat_tagged = {p: [r.latency for r in reqs if r.precision == p] for p in (2, 3)}
-    base = [S.Request(**{f.name: getattr(r, f.name) for f in dataclasses.fields(S.Request)
-                         if f.name not in ("admitted", "finished", "result")}) for r in reqs]
-    plain = S.run_one_at_a_time(base, precision=2)
-    same = sum(int(np.array_equal(served[b.rid].view(np.uint64), b.result.view(np.uint64)))
-               for b in base if b.precision == 2)
-    n64 = sum(1 for b in base if b.precision == 2)
+    variety = not args.plain_mix
+    points = {}
+    for qps in [float(q) for q in str(args.qps).split(",")]:
+        reqs = S.request_mix(args.requests, qps=qps, seed=2, variety=variety)
+        srv = S.Server(budget_bytes=200e9, max_active=16, chunk_steps=200)
+        stats = srv.run(reqs)
+        served = {r.rid: r.result for r in reqs}
+        lat_tagged = {p: [r.latency for r in reqs if r.precision == p] for p in (2, 3)}
+        base = [S.Request(**{f.name: getattr(r, f.name) for f in dataclasses.fields(S.Request)
+                             if f.name not in ("admitted", "finished", "result")}) for r in reqs]
+        plain = S.run_one_at_a_time(base, precision=2)
+        same = sum(int(np.array_equal(served[b.rid].view(np.uint64), b.result.view(np.uint64)))
+                   for b in base if b.precision == 2)
+        n64 = sum(1 for b in base if b.precision == 2)
+        points[qps] = {"server": stats, "one_at_a_time_fp64": plain,
+                       "latency_p50_s": stats["latency_p50_s"], "latency_p99_s": stats["latency_p99_s"],
+                       "latency_p50_by_precision": {("fp64" if p == 2 else "fp32c"): float(np.percentile(v, 50))
+                                                    for p, v in lat_tagged.items() if v},
+                       "msd_requests": sum(1 for r in reqs if r.BC == 2),
+                       "trap_requests": sum(1 for r in reqs if r.make_V is not None),
+                       "fp64_requests_identical_to_one_at_a_time": f"{same}/{n64}"}
+        print(f"serving at {qps} QPS: p50 {stats['latency_p50_s']:.3f} s, p99 {stats['latency_p99_s']:.3f} s",
+              file=sys.stderr, flush=True)
+    q0 = next(iter(points))
+    first = points[q0]["server"]
+    rates = ", ".join(f"{q:g}" for q in points)
     line = {
         "metric": "RK4 grid-point updates/s & % HBM peak; batched-solve p50/p99 latency",
-        "value": stats["throughput"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": len(reqs), "warmup": 1,
-        "ms_per_step": stats["makespan_s"] * 1e3 / len(reqs), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64 + f32c (per request)", "data": "synthetic",
-        "config": {"workload": f"serving: {len(reqs)} Poisson requests at {args.qps} QPS (seed 2), mixed 1D/2D/3D "
-                               "CD/2SHOC FP64/FP32c, 100-20000 steps, 200 GB budget, initial states built on the "
-                               "device from parameters",
-                   "server": stats, "one_at_a_time_fp64": plain,
-                   "latency_p50_s": stats["latency_p50_s"], "latency_p99_s": stats["latency_p99_s"],
-                   "latency_p50_by_precision": {("fp64" if p == 2 else "fp32c"): float(np.percentile(v, 50))
-                                                for p, v in lat_tagged.items() if v},
-                   "fp64_requests_identical_to_one_at_a_time": f"{same}/{n64}"},
+        "value": first["throughput"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": 1, "ms_per_step": first["makespan_s"] * 1e3 / args.requests, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64 + f32c (per request)", "data": "synthetic",
+        "config": {"workload": f"serving: {args.requests} Poisson requests at {rates} QPS (seed 2), mixed 1D/2D/3D "
+                               "CD/2SHOC FP64/FP32c, 100-20000 steps, "
+                               + ("Dirichlet or MSD, half with a harmonic trap, " if variety else "")
+                               + "200 GB budget, initial states built on the device from parameters "
+                               f"(value: the {q0:g} QPS stream)",
+                   "qps_points": {f"{q:g}": v for q, v in points.items()},
+                   "latency_p50_s": points[q0]["latency_p50_s"], "latency_p99_s": points[q0]["latency_p99_s"]},
     }
     print(json.dumps(line))
 
@@ -689,7 +706,9 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="pick fusion and tile with the built-in tuner")
     ap.add_argument("--batch-config", default="", metavar="F,KIND,ROWS,GRID",
                     help="batched: stages per launch, kernel kind, rows, grid (skips the tuner)")
-    ap.add_argument("--qps", type=float, default=4.0, help="serving: Poisson arrival rate")
+    ap.add_argument("--qps", default="4,16", help="serving: Poisson arrival rates, comma-separated")
+    ap.add_argument("--plain-mix", action="store_true",
+                    help="serving: Dirichlet, V = 0 requests only (no MSD or trap requests)")
     ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
     ap.add_argument("--workload", default="2d",
                     choices=("2d", "2d-matrix", "batched", "1d", "3d", "3d-counters", "serving", "vorticity",
