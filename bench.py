@@ -620,7 +620,8 @@ def run_3d_counters(args):
     U, h = vortex_rings_3d()
     k = (0.8 if args.method == 1 else 0.8 * 0.75) * h * h / (3 * np.sqrt(2))
     sol = nm.Solver(U.shape, args.method, args.precision, 1, -1.0, 1.0, h * h, k)
-    sol.set_fusion(args.fusion)
+    if args.fusion is not None:
+        sol.set_fusion(args.fusion)
     sol.set_kernel(args.kernel, args.tile or 4)
     sol.set_state(U, np.zeros(U.shape))
     sol.step(args.steps)
@@ -693,7 +694,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--graph", type=int, default=CHUNK, help="RK4 steps per hipGraph replay (0 = off)")
-    ap.add_argument("--fusion", type=int, default=2, choices=(1, 2, 4),
+    ap.add_argument("--fusion", type=int, default=None, choices=(1, 2, 4),
                     help="RK4 stages per kernel launch (results are identical)")
     ap.add_argument("--precision", type=int, default=2, choices=(1, 2, 3),
                     help="2d workload: 2 FP64 (the headline), 1 FP32, 3 compensated FP32")
@@ -759,10 +760,10 @@ def main():
     k = K_RULE * H * H
     sol = nm.Solver((N, M), 2, args.precision, 2, -1.0, 1.0, H * H, k)
     sol.set_graph(args.graph)
-    sol.set_fusion(args.fusion)
-    if args.tile is None:
-        args.tile = 8
-    sol.set_kernel(args.kernel, args.tile)
+    if args.fusion is not None:
+        sol.set_fusion(args.fusion)
+    if args.tile is not None or args.kernel != 1:
+        sol.set_kernel(args.kernel, args.tile or 8)
     sol.set_state(U, V)
     if args.autotune:
         sol.autotune(5)
@@ -790,12 +791,18 @@ def main():
     bytes_launch = alg_bytes * N * M / (4 // args.fusion)
     achieved = bytes_launch / t_launch / 1e9
     dbytes = design_bytes(args.precision, args.fusion)
-    traffic = None
+    traffic = valu = None
     if os.path.exists(PROFILE_TRAFFIC) and args.precision == 2:
         with open(PROFILE_TRAFFIC) as f:
             prof = json.load(f).get(f"fusion{args.fusion}", {})
         if prof.get("tile") == args.tile and prof.get("kernel", 0) == args.kernel:
             traffic = prof.get("hbm_bytes_per_launch")
+            if prof.get("valu_insts_per_launch"):
+                # VALU issue: wave64 instructions (FP64 or 32-bit) issue at one
+                # per CU per clock; 256 CUs at 2.4 GHz
+                cap = 256 * 2.4e9 * t_launch
+                valu = {"insts_per_launch": prof["valu_insts_per_launch"], "issue_frac_at_2p4GHz":
+                        round(prof["valu_insts_per_launch"] / cap, 4)}
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kernel_name(args.fusion, args.kernel, args.tile),
@@ -804,7 +811,8 @@ def main():
                 "design_bytes_per_point_step": dbytes,
                 "design_GBps": round(dbytes * N * M / (t_launch * (4 // args.fusion)) / 1e9, 1),
                 "design_frac": round(dbytes * N * M / (t_launch * (4 // args.fusion)) / 1e9 / HBM_PEAK_GBS, 4),
-                "launch_us_event_per_kernel": [round(x * 1e3, 2) for x in sol.profile_launches(20)]}
+                "launch_us_event_per_kernel": [round(x * 1e3, 2) for x in sol.profile_launches(20)],
+                "valu": valu}
     sol.close()
 
     dropin = dropin_host_rate(nm, U, V, k) if args.precision == 2 else None

@@ -245,16 +245,16 @@ int nlse_vortex_profile(int m, double OM, double a, double s, double hr, int n, 
  * not available for 1D N <= 4096, which runs whole chunks in one kernel) */
 int nlse_profile_launches(nlse_solver *h, int steps, double *ms_launch, int *nlaunch);
 /* RK4 stages fused per kernel launch (the results are identical for every
- * choice): 2D 1, 2 or 4 (default 2); 3D 1 or 2 (two stages per
- * plane-marching launch, CD and 2SHOC, with a boundary other than MSD; MSD
- * runs one stage per launch; default 2, except 1 for 2SHOC in double
- * precision, where the pair kernel is slower) */
+ * choice): 2D 1, 2 or 4 (default 4, a whole step per launch); 3D 1 or 2
+ * (default 2: two stages per plane-marching launch, CD and 2SHOC, with a
+ * boundary other than MSD and planes of fewer than 2^31 points; MSD runs one
+ * stage per launch) */
 int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
 /* 2D: the stage kernel family and its rows per tile (results identical for
  * every choice):
  *   kind 0  LDS tiles of 64 columns x 8, 16 or 32 rows
  *   kind 1  row-sweep LDS tiles (one lane per column, DPP neighbours),
- *           8 (default), 12 or 16 rows
+ *           8, 12 or 16 (default) rows
  *   kind 2  register-marching strips (one wave per strip, no LDS), segments
  *           of 3..256 rows, for 1, 2 or 4 stages per launch
  * 3D 2SHOC, one stage per launch: kind 1 (default) marches planes with D
@@ -326,8 +326,8 @@ int nlse_batch_autotune(nlse_batch *b, int steps, int *best_fusion, int *best_ti
 /* Groups of resident 2D solvers (no counterpart in the reference, which
  * solves one problem per call): the members (2D, one method and precision;
  * any sizes, parameters and boundaries) are advanced together by the batched
- * kernels, one persistent launch per pair of RK4 stages over the tiles of
- * every member, each in its own buffers.  Results are bit-identical to
+ * kernels, one persistent launch per RK4 step over the 16-row tiles of every
+ * member, each in its own buffers.  Results are bit-identical to
  * stepping each member alone.  nlse_group_create waits for the members'
  * queued work; nlse_group_step enqueues on the group's own stream (returns
  * at once); nlse_group_done returns 1 when the group's work has finished, 0

@@ -103,6 +103,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
 {
     using Cfg = RowsCfg<METHOD, FIRST, LAST, TI, NW>;
     constexpr int R = Cfg::R, NF = Cfg::NF, H = Cfg::H, W = Cfg::W, TE = Cfg::TE, RW = Cfg::RW;
+    static_assert(TE <= 32, "the MSD column pass gives each extended row one lane of a half wave");
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     __builtin_assume(w >= 0 && w < NW);

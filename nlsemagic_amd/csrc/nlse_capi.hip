@@ -508,11 +508,14 @@ int nlse_create(nlse_solver** out, int dim, int N, int M, int L, int method, int
         return fail(NLSE_EHIP, "stream/event creation failed");
     }
     h->ubase = h->U;
-    // 2D: pairs of stages; 3D: pairs where the BC allows
-    h->fuse = (dim == 1) ? 1 : 2;
+    // 2D: a whole RK4 step per launch on row-sweep tiles of 16 rows (on a
+    // 1024^2 MSD grid 8-55 % faster than pairs of stages on 8-row tiles,
+    // across methods and precisions: half the launches, each with a fixed
+    // start-up and drain); 3D: pairs of stages where the BC allows
+    h->fuse = (dim == 1) ? 1 : (dim == 2 ? 4 : 2);
     h->kern = KERNEL_ROWS;         // the fastest family on a 1024^2 grid and for 3D 2SHOC
     // 3D: plane-march rows; 1D: steps per launch (the fastest on N = 2048)
-    h->tile = (dim == 3) ? 4 : (dim == 1 ? (method == 2 ? 16 : 32) : 8);
+    h->tile = (dim == 3) ? 4 : (dim == 1 ? (method == 2 ? 16 : 32) : 16);
     set_params(h, BC, s, a, h2, k);
     // a defined initial state: zero field and potential, cleared on the
     // solver's own stream (a null-stream memset is not ordered with it and
@@ -1128,12 +1131,12 @@ struct nlse_group {
 
 namespace {
 
-constexpr int GROUP_ROWS = 8;  // row-sweep tiles of 8 rows, two stages per launch
+constexpr int GROUP_ROWS = 16;  // row-sweep tiles of 16 rows, a whole RK4 step per launch
 
 template <typename T>
 int group_table_t(nlse_group* g)
 {
-    const int R = g->method == 2 ? 2 : 1, TW = 64 - 4 * R;
+    const int R = g->method == 2 ? 2 : 1, TW = 64 - 8 * R;  // owned columns of a four-stage tile
     std::vector<BatchProb2<T>> P(g->mem.size());
     std::vector<int> tp;
     int t0 = 0;
@@ -1192,14 +1195,13 @@ int group_launch_t(nlse_group* g, int first, int last, int r_yin, int r_out)
     return NLSE_OK;
 }
 
-// one RK4 step of every member: stages 1-2 into buffer 1 (Y_2, ktot), 3-4
-// into the buffer not holding U, which then holds U
+// one RK4 step of every member, one launch: from the buffer holding U into
+// the one that does not, which then holds U
 int group_step1(nlse_group* g)
 {
     const int cur = g->parity ? 2 : 0, other = 2 - cur;
     const bool d = g->prec == 2;
-    int rc = d ? group_launch_t<double>(g, 1, 2, cur, 1) : group_launch_t<float>(g, 1, 2, cur, 1);
-    if (rc == NLSE_OK) rc = d ? group_launch_t<double>(g, 3, 4, 1, other) : group_launch_t<float>(g, 3, 4, 1, other);
+    const int rc = d ? group_launch_t<double>(g, 1, 4, cur, other) : group_launch_t<float>(g, 1, 4, cur, other);
     if (rc == NLSE_OK) g->parity ^= 1;
     return rc;
 }
