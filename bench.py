@@ -290,6 +290,10 @@ def run_2d_matrix(args):
         for prec in (2, 1, 3):
             sol = nm.Solver((N, M), m, prec, 2, -1.0, 1.0, H * H, kh2 * H * H)
             sol.set_graph(CHUNK)
+            if args.fusion is not None:
+                sol.set_fusion(args.fusion)
+            if args.tile is not None or args.kernel != 1:
+                sol.set_kernel(args.kernel, args.tile or 8)
             sol.set_state(U, V)
             sol.step(2 * CHUNK)
             sol.sync()
