@@ -262,8 +262,8 @@ int nlse_set_fusion(nlse_solver *h, int stages_per_launch);
  * pass and an F pass per stage.  MSD boundaries always use the two passes.
  * Two stages per launch (nlse_set_fusion) march tiles of 4 rows x 56
  * columns (2SHOC) or 4 x 60 (CD) whatever the kind.
- * 1D: kind 1 (default) advances `rows` steps (1..96 CD, 1..48 2SHOC; default
- * 32 CD, 16 2SHOC) per launch on 256-point windows with a halo, one per CU;
+ * 1D: kind 1 (default) advances `rows` steps (1..80 CD, 1..32 2SHOC; default
+ * 32 CD, 16 2SHOC) per launch on windows of 64-256 points with a halo, one per CU;
  * kind 0 runs a whole chunk in one workgroup (N <= 4096) or one launch per
  * stage. */
 int nlse_set_kernel(nlse_solver *h, int kind, int rows);

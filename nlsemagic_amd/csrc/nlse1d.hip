@@ -12,11 +12,15 @@
 //     This is the launch-free form for the small 1D problems the reference
 //     targets (N = 2048): a grid-wide launch per stage would be latency bound;
 //   - block1d_kernel (the default): S whole RK4 steps per launch, one
-//     workgroup per segment of 256 points, each on its own CU.  A workgroup
-//     loads its segment plus a halo of 4*R*S points (R = 1 CD, 2 2SHOC: the
-//     reach of one stage), runs the S steps on chip like the resident kernel,
-//     and writes back its own points; values near the window edge go stale
-//     by R points per stage and never reach the owned points.  The boundary
+//     workgroup per segment of 64, 128 or 256 points (the largest that still
+//     gives 128 segments, or 64), each on its own CU.  A workgroup loads its
+//     segment plus a halo of 4*R*S points (R = 1 CD, 2 2SHOC: the reach of
+//     one stage), runs the S steps on chip like the resident kernel, and
+//     writes back its own points; values near the window edge go stale by R
+//     points per stage and never reach the owned points.  A wave whose
+//     points all lie more than R * (stages left) from the segment skips the
+//     stage: the owned points depend on nothing that far out, and no point
+//     they depend on reads it.  The boundary
 //     formulas reach up to three points inward in one stage (MSD through the
 //     partner's stencil, the one-sided BC 4 difference), further than R, so
 //     the segment origins shift back until the last segment owns at least
@@ -263,86 +267,126 @@ static hipError_t launch_res(c2<T>* u, c2<T>* ulo, const T* V, Coefs<T> c, int N
 
 // ------------------------------------------ time-blocked windows, many CUs
 
-constexpr int BLK_SEG = 256;  // owned points per workgroup
+constexpr int BLK_SEG = 256;  // the largest segment: owned points per workgroup
 
+// One wave per 64 consecutive points: a core of CW = 64 - 2 HW points and a
+// halo of HW = 4 R points (one RK4 step's reach) each side, copies of the
+// neighbouring waves' core points.  A wave runs the four stages of a step on
+// its own: the stage input (and D) of its 64 points goes to a wave-private
+// LDS slice and the stencils (with the reference's boundary rules) read it
+// there, so no workgroup barrier is needed within a step (neighbours by DPP
+// instead ran 6 % slower: the lane shifts of FP64 values cost more than the
+// LDS round trip).  The halo lanes go stale by R points per stage and
+// are refreshed from the neighbours' cores (U and U_lo through LDS, one
+// workgroup barrier) at the end of every step.  The window's outer halo of
+// 4 R S points covers the S steps of the launch as in the resident kernel.
 template <typename T, int METHOD, bool COMP>
 __global__ __launch_bounds__(1024) void block1d_kernel(const c2<T>* __restrict__ uin, c2<T>* __restrict__ uout,
                                                        const c2<T>* __restrict__ loin, c2<T>* __restrict__ loout,
                                                        const T* __restrict__ Vg, Coefs<T> c, int N, int BC,
-                                                       int steps, int halo, int soff)
+                                                       int steps, int halo, int seg, int soff)
 {
-    // stage input and D double-buffered by stage parity: a buffer is
-    // rewritten two stages later, after every thread has passed the barrier
-    // of the stage in between, so one barrier per stage (two for 2SHOC)
-    // separates each write from its readers
-    __shared__ c2<T> sY[2][1024 + 2];
-    __shared__ c2<T> sD[(METHOD == 2) ? 2 : 1][(METHOD == 2) ? 1024 + 2 : 1];
-    const int o0 = max((int)blockIdx.x * BLK_SEG - soff, 0), o1 = min((int)(blockIdx.x + 1) * BLK_SEG - soff, N);
+    constexpr int R = (METHOD == 2) ? 2 : 1;
+    constexpr int HW = 4 * R, CW = 64 - 2 * HW;
+    constexpr int PAD = 4;  // the boundary formulas reach three points inward
+    // wave-private stage input and D (padding lanes hold values no core
+    // point depends on); window-indexed U and U_lo of the cores, by step parity
+    __shared__ c2<T> sW[16][64 + 2 * PAD];
+    __shared__ c2<T> sWD[(METHOD == 2) ? 16 : 1][(METHOD == 2) ? 64 + 2 * PAD : 1];
+    __shared__ c2<T> sX[2][16 * CW];
+    __shared__ c2<T> sXL[COMP ? 2 : 1][COMP ? 16 * CW : 1];
+    const int o0 = max((int)blockIdx.x * seg - soff, 0), o1 = min((int)(blockIdx.x + 1) * seg - soff, N);
     const int w0 = max(o0 - halo, 0), w1 = min(o1 + halo, N);
-    const int t = threadIdx.x;
-    const int i = w0 + t;
-    const bool act = i < w1;
-    const int ic = act ? i : w1 - 1;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int nw = (w1 - w0 + CW - 1) / CW;  // waves with a core in the window
+    const int ib = w0 + w * CW - HW;         // point of lane 0
+    const int i = ib + lane;
+    const bool core = lane >= HW && lane < 64 - HW && i < w1;
+    const int ic = i < 0 ? 0 : (i >= N ? N - 1 : i);
     c2<T> U = uin[ic], Lo = COMP ? loin[ic] : mk<T>(0, 0), A = mk<T>(0, 0), Y = U;
     const T Vr = Vg[ic];
     auto Vl = [&](int gi) { return gi == i ? Vr : Vg[gi]; };
-    // one padding element each side of the window (stale values beyond the
-    // window only reach points within the halo)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-        if (t == 0) {
-            sY[b][0] = mk<T>(0, 0);
-            if (METHOD == 2) sD[b][0] = mk<T>(0, 0);
-        }
-        if (t == w1 - w0) {
-            sY[b][t + 1] = mk<T>(0, 0);
-            if (METHOD == 2) sD[b][t + 1] = mk<T>(0, 0);
-        }
-    }
+    auto Yl = [&](int gi) { return sW[w][gi - ib + PAD]; };
+    auto Dl = [&](int gi) { return sWD[METHOD == 2 ? w : 0][gi - ib + PAD]; };
+    auto rhs = make_rhs1d<T, METHOD>(Yl, Vl, Dl, c, N, BC);
+    const bool ingrid = i >= 0 && i < N;
+    // distance of this wave's nearest point from the segment (uniform)
+    const int whi = ib + 63;
+    const int wdist = whi < o0 ? o0 - whi : (ib > o1 - 1 ? ib - (o1 - 1) : 0);
+    const int G = 4 * steps;  // stages in this launch
     for (int step = 0; step < steps; ++step) {
         static_for<0, 4>([&](auto qc) {
             constexpr int stage = decltype(qc)::value + 1;
-            constexpr int b = (stage - 1) & 1;
-            auto Yl = [&](int gi) { return sY[b][gi - w0 + 1]; };
-            auto Dl = [&](int gi) { return sD[METHOD == 2 ? b : 0][gi - w0 + 1]; };
-            auto rhs = make_rhs1d<T, METHOD>(Yl, Vl, Dl, c, N, BC);
-            if (act) sY[b][t + 1] = Y;
-            __syncthreads();
+            // this stage's output is needed within R * (G - 1 - g) of the
+            // segment, its input within R * (G - g); waves beyond that, and
+            // waves with no core in the window, skip the stage
+            const bool live = w < nw && wdist <= R * (G - (4 * step + stage - 1));
+            if (!live) return;
+            sW[w][PAD + lane] = Y;
+            __builtin_amdgcn_wave_barrier();
             if constexpr (METHOD == 2) {
-                if (act) sD[b][t + 1] = rhs.d_at(i);
-                __syncthreads();
+                const c2<T> d = ingrid ? rhs.d_at(i) : mk<T>(0, 0);
+                sWD[w][PAD + lane] = d;
+                __builtin_amdgcn_wave_barrier();
             }
-            if (act) {
-                const c2<T> f = rhs.f_at(i);
-                if constexpr (stage == 1) {
-                    A = f;
-                    Y = mk<T>(U.x + c.k2 * f.x, U.y + c.k2 * f.y);
-                } else if constexpr (stage == 4) {
-                    A.x = A.x + (T)1.0 * f.x;
-                    A.y = A.y + (T)1.0 * f.y;
-                    if constexpr (COMP) {
-                        comp_add<T>(U.x, Lo.x, c.k6 * A.x);
-                        comp_add<T>(U.y, Lo.y, c.k6 * A.y);
-                    } else {
-                        U = mk<T>(U.x + c.k6 * A.x, U.y + c.k6 * A.y);
-                    }
-                    Y = U;
+            const c2<T> f = ingrid ? rhs.f_at(i) : mk<T>(0, 0);
+            if constexpr (stage == 1) {
+                A = f;
+                Y = mk<T>(U.x + c.k2 * f.x, U.y + c.k2 * f.y);
+            } else if constexpr (stage == 4) {
+                A.x = A.x + (T)1.0 * f.x;
+                A.y = A.y + (T)1.0 * f.y;
+                if constexpr (COMP) {
+                    comp_add<T>(U.x, Lo.x, c.k6 * A.x);
+                    comp_add<T>(U.y, Lo.y, c.k6 * A.y);
                 } else {
-                    A.x = A.x + (T)2.0 * f.x;
-                    A.y = A.y + (T)2.0 * f.y;
-                    const T kk = (stage == 2) ? c.k2 : c.k;
-                    Y = mk<T>(U.x + kk * f.x, U.y + kk * f.y);
+                    U = mk<T>(U.x + c.k6 * A.x, U.y + c.k6 * A.y);
                 }
+            } else {
+                A.x = A.x + (T)2.0 * f.x;
+                A.y = A.y + (T)2.0 * f.y;
+                const T kk = (stage == 2) ? c.k2 : c.k;
+                Y = mk<T>(U.x + kk * f.x, U.y + kk * f.y);
             }
+            __builtin_amdgcn_wave_barrier();  // the slices are read before the next stage rewrites them
         });
+        // refresh the halo lanes from the neighbouring waves' cores
+        const int par = step & 1;
+        if (core && w < nw) {
+            sX[par][i - w0] = U;
+            if constexpr (COMP) sXL[par][i - w0] = Lo;
+        }
+        __syncthreads();
+        const int x = i - w0;
+        if (!core && w < nw && x >= 0 && x < nw * CW && i < w1) {
+            U = sX[par][x];
+            if constexpr (COMP) Lo = sXL[par][x];
+        }
+        Y = U;
     }
-    if (act && i >= o0 && i < o1) {
+    if (core && i >= o0 && i < o1) {
         uout[i] = U;
         if (COMP) loout[i] = Lo;
     }
 }
 
-int block1d_max_steps(int method) { return (1024 - BLK_SEG) / (2 * 4 * (method == 2 ? 2 : 1)); }
+// steps per launch: the window (a segment of up to BLK_SEG points and a halo
+// of 4 R S each side) must fit the cores of 16 waves
+int block1d_max_steps(int method)
+{
+    const int R = (method == 2) ? 2 : 1;
+    return (16 * (64 - 8 * R) - BLK_SEG) / (2 * 4 * R);
+}
+
+// owned points per workgroup: the largest of 256, 128, 64 that gives at
+// least 128 segments, else 64
+static int block1d_segment(int N)
+{
+    int seg = BLK_SEG;
+    while (seg > 64 && N / seg < 128) seg /= 2;
+    return seg;
+}
 
 template <typename T>
 hipError_t launch_block1d(const c2<T>* uin, c2<T>* uout, const c2<T>* loin, c2<T>* loout, const T* V, Coefs<T> c,
@@ -350,28 +394,30 @@ hipError_t launch_block1d(const c2<T>* uin, c2<T>* uout, const c2<T>* loin, c2<T
 {
     const int R = (method == 2) ? 2 : 1;
     const int halo = 4 * R * halo_steps;
-    const int nb = (N + BLK_SEG - 1) / BLK_SEG;
-    // segment b owns [b*BLK_SEG - soff, (b+1)*BLK_SEG - soff) clipped to [0, N):
+    const int seg = block1d_segment(N);
+    const int nb = (N + seg - 1) / seg;
+    // segment b owns [b*seg - soff, (b+1)*seg - soff) clipped to [0, N):
     // the last segment owns at least 4 points (see block1d_kernel)
-    const int tail = N % BLK_SEG;
+    const int tail = N % seg;
     const int soff = (nb > 1 && tail > 0 && tail < 4) ? 4 - tail : 0;
-    const int win = std::min(N, BLK_SEG + 2 * halo);
-    const dim3 blk(((win + 63) / 64) * 64);
+    const int win = std::min(N, seg + 2 * halo);
+    const int CW = 64 - 8 * R;  // core points per wave
+    const dim3 blk(((win + CW - 1) / CW) * 64);
     const bool comp = loin != nullptr;
     if (method == 2) {
         if (comp)
             hipLaunchKernelGGL((block1d_kernel<T, 2, true>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
-                               BC, steps, halo, soff);
+                               BC, steps, halo, seg, soff);
         else
             hipLaunchKernelGGL((block1d_kernel<T, 2, false>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
-                               BC, steps, halo, soff);
+                               BC, steps, halo, seg, soff);
     } else {
         if (comp)
             hipLaunchKernelGGL((block1d_kernel<T, 1, true>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
-                               BC, steps, halo, soff);
+                               BC, steps, halo, seg, soff);
         else
             hipLaunchKernelGGL((block1d_kernel<T, 1, false>), dim3(nb), blk, 0, st, uin, uout, loin, loout, V, c, N,
-                               BC, steps, halo, soff);
+                               BC, steps, halo, seg, soff);
     }
     return hipGetLastError();
 }

@@ -30,25 +30,6 @@
 
 namespace nlse {
 
-// DPP lane shifts over the whole 64-lane wave (GFX9 wavefront shifts)
-constexpr int DPP_NEXT = 0x130;  // wave_shl:1 -- lane l reads lane l+1 (column j+1)
-constexpr int DPP_PREV = 0x138;  // wave_shr:1 -- lane l reads lane l-1 (column j-1)
-
-template <int CTRL> __device__ __forceinline__ float lane_shift(float v)
-{
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
-}
-template <int CTRL> __device__ __forceinline__ double lane_shift(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xf, 0xf, true);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, true);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-template <int CTRL, typename T> __device__ __forceinline__ c2<T> lane_shift2(c2<T> v)
-{
-    return mk<T>(lane_shift<CTRL>(v.x), lane_shift<CTRL>(v.y));
-}
 
 template <int METHOD, int FIRST, int LAST>
 struct StripCfg {

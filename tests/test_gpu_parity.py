@@ -349,18 +349,18 @@ def test_march3d_compensated_matches_two_pass(hip, shape, BC):
 
 def _block1d_cases():
     """1D time-blocked windows: grids smaller than, equal to and larger than
-    one 256-point segment, steps per launch 1..max (CD 96, 2SHOC 48), chunk
-    lengths that split into odd and even launch counts, every BC."""
+    one segment, steps per launch 1..max (CD 80, 2SHOC 32), chunk lengths that
+    split into odd and even launch counts, every BC."""
     out = []
     seed = 30000
     for n in (5, 256, 257, 700, 2048, 5000):
         for method in (1, 2):
             for BC in (1, 2, 3, 4, 0):
                 seed += 1
-                S = (1, 3, 8, 17, 48 if method == 2 else 96)[seed % 5]
+                S = (1, 3, 8, 17, 32 if method == 2 else 80)[seed % 5]
                 steps = (1, 5, 37, 100)[seed % 4]
                 out.append((n, method, BC, S, steps, (2, 1, 3)[seed % 3], seed))
-    # the last 256-point segment would own one to three points: the MSD and
+    # the last segment would own one to three points: the MSD and
     # one-sided boundary formulas reach further inward than a stage's R, and
     # with one or two steps per launch that reach meets the window's stale edge
     for n, BC in ((257, 2), (258, 4), (257, 4), (259, 4), (513, 2)):
