@@ -8,11 +8,14 @@
 //     in LDS;
 //   - extended row e belongs to wave e mod NW in every phase, which keeps the
 //     row's Y, U, V and ktot in that wave's registers; the rows are loaded
-//     all at once at the start (every load in flight together);
-//   - j +- 1 neighbours of a row's own values come from adjacent lanes (DPP
-//     wave shifts), the rows i +- 1 (and the diagonal corners of the 2SHOC
-//     stencil) from the LDS copy of the stage input, and for 2SHOC the
-//     D = Lap_CD(Y)/h^2 rows i +- 1 from an LDS copy of D;
+//     all at once at the start, stage input first (every load in flight
+//     together; rows outside the grid load a clamped copy, so every wave
+//     issues the same loads);
+//   - the rows i +- 1, the diagonal corners of the 2SHOC stencil and a row's
+//     own j +- 1 neighbours come from the LDS copy of the stage input (lane
+//     +- 1 of a row's copy costs no VALU moves, unlike DPP shifts of FP64
+//     values), and for 2SHOC D = Lap_CD(Y)/h^2 of rows i +- 1 and columns
+//     j +- 1 from an LDS copy of D;
 //   - per stage: D phase, barrier, F phase (each row's F and RK4 update at
 //     once, the new Y in the row's registers), barrier, new Y rows written
 //     to LDS, barrier; the last stage writes its rows to HBM.
@@ -23,8 +26,9 @@
 // wave holds (row, side) pairs in its lanes and computes every boundary
 // column point's D from the partner column's values in LDS (in-lane, each
 // row would spend a full wave of instructions, a division among them, on two
-// useful lanes).  Every owned point gets the reference's operations in its
-// operand order.
+// useful lanes); the MSD column's F partner comes by DPP from the adjacent
+// lane.  Every owned point gets the reference's operations in its operand
+// order.
 #pragma once
 
 #include "nlse2d_strip.h"
