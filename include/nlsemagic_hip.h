@@ -111,6 +111,9 @@ int nlse_step(nlse_solver *h, int steps);
 /* enqueue `steps` steps between two HIP events on the solver's stream,
  * wait, and return the elapsed device time in milliseconds */
 int nlse_step_timed(nlse_solver *h, int steps, double *ms);
+/* waits like nlse_sync, the calling thread asleep rather than spinning (for
+ * hosts that wait on many solvers from several threads) */
+int nlse_wait(nlse_solver *h);
 int nlse_sync(nlse_solver *h);
 /* non-blocking: 1 when all work enqueued on the solver's stream is done,
  * 0 while it runs, a negative NLSE_E* code on error */
@@ -338,6 +341,9 @@ typedef struct nlse_group nlse_group;
 int nlse_group_create(nlse_group **g, nlse_solver *const *members, int n);
 int nlse_group_step(nlse_group *g, int steps);
 int nlse_group_done(nlse_group *g);
+/* waits until the group's enqueued work has finished, the calling thread
+ * asleep rather than spinning */
+int nlse_group_sync(nlse_group *g);
 void nlse_group_destroy(nlse_group *g);
 
 /* measurement helper: mean device ms of a plain streaming copy of `bytes`

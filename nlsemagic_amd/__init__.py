@@ -139,6 +139,10 @@ def lib():
     L.nlse_group_step.argtypes = [C.c_void_p, C.c_int]
     L.nlse_group_done.restype = C.c_int
     L.nlse_group_done.argtypes = [C.c_void_p]
+    L.nlse_group_sync.restype = C.c_int
+    L.nlse_group_sync.argtypes = [C.c_void_p]
+    L.nlse_wait.restype = C.c_int
+    L.nlse_wait.argtypes = [C.c_void_p]
     L.nlse_group_destroy.restype = None
     L.nlse_group_destroy.argtypes = [C.c_void_p]
     L.nlse_diagnostics_noalloc.restype = C.c_int
@@ -500,6 +504,10 @@ class Solver:
     def sync(self):
         _check(self._lib.nlse_sync(self._h))
 
+    def wait(self):
+        """sync() with the calling thread asleep instead of spinning."""
+        _check(self._lib.nlse_wait(self._h))
+
     def done(self):
         """True when every step enqueued so far has finished (non-blocking)."""
         rc = self._lib.nlse_query(self._h)
@@ -533,6 +541,10 @@ class Group:
         if rc < 0:
             _check(-rc)
         return bool(rc)
+
+    def sync(self):
+        """Wait until the group's enqueued steps have finished."""
+        _check(self._lib.nlse_group_sync(self._h))
 
     def close(self):
         if self._h:

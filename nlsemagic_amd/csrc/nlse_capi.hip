@@ -885,6 +885,25 @@ int nlse_sync(nlse_solver* h)
     return NLSE_OK;
 }
 
+// waits for everything enqueued on `st` so far with the calling thread
+// asleep (a blocking-sync event) rather than spinning
+static int wait_blocking(hipStream_t st)
+{
+    hipEvent_t ev;
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventBlockingSync | hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, st);
+    if (e == hipSuccess) e = hipEventSynchronize(ev);
+    (void)hipEventDestroy(ev);
+    if (e != hipSuccess) return fail(NLSE_EHIP, std::string("blocking wait: ") + hipGetErrorString(e));
+    return NLSE_OK;
+}
+
+int nlse_wait(nlse_solver* h)
+{
+    if (!h) return fail(NLSE_EINVAL, "null solver");
+    return wait_blocking(h->stream);
+}
+
 int nlse_set_fusion(nlse_solver* h, int stages_per_launch)
 {
     if (!h) return fail(NLSE_EINVAL, "null solver");
@@ -1272,6 +1291,12 @@ int nlse_group_done(nlse_group* g)
     if (e == hipSuccess) return 1;
     if (e == hipErrorNotReady) return 0;
     return -fail(NLSE_EHIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int nlse_group_sync(nlse_group* g)
+{
+    if (!g) return fail(NLSE_EINVAL, "null group");
+    return wait_blocking(g->stream);
 }
 
 void nlse_group_destroy(nlse_group* g)

@@ -33,7 +33,9 @@ chunk in a loop (src/NLSEmagic2D/NLSEmagic2D.m:251-289).
 from __future__ import annotations
 
 import dataclasses
+import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from collections import deque
 from typing import Callable, List, Optional
 
@@ -119,6 +121,30 @@ class Server:
         return sol
 
     def run(self, requests: List[Request]) -> dict:
+        # Waiter threads sleep on each enqueued chunk's completion (a
+        # blocking-sync event; the library call releases the GIL) and wake
+        # the loop, which otherwise sleeps until the next arrival; every
+        # waiter of a solver or group is joined before that solver or group
+        # is closed.
+        wake = threading.Event()
+        pool = ThreadPoolExecutor(max_workers=self.max_active + 4)
+        waiters = {}  # id(solver or group) -> futures of its waiters
+
+        def watch(obj):
+            f = pool.submit(obj.wait if isinstance(obj, Solver) else obj.sync)
+            f.add_done_callback(lambda _: wake.set())
+            waiters.setdefault(id(obj), []).append(f)
+
+        def settle(obj):
+            for f in waiters.pop(id(obj), []):
+                f.result()
+
+        try:
+            return self._run(requests, wake, watch, settle)
+        finally:
+            pool.shutdown(wait=True)
+
+    def _run(self, requests, wake, watch, settle) -> dict:
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         waiting: List[Request] = []
         active = {}  # rid -> [request, solver, steps_left, footprint]
@@ -129,6 +155,7 @@ class Server:
         t0 = time.perf_counter()
         clock = lambda: time.perf_counter() - t0
         while pending or waiting or active or groups:
+            wake.clear()
             now = clock()
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
@@ -147,6 +174,7 @@ class Server:
             progressed = False
             # running groups: a finished one hands its members back
             for g in [g for g in groups if g.done()]:
+                settle(g)
                 g.close()
                 groups.remove(g)
                 for rid in grouped.pop(id(g)):
@@ -167,6 +195,7 @@ class Server:
                     else:
                         sol.sync()
                     r.finished = clock()
+                    settle(sol)
                     sol.close()
                     used -= fp
                     del active[rid]
@@ -176,6 +205,7 @@ class Server:
                     continue
                 n = min(self.chunk, left)
                 sol.step(n)
+                watch(sol)
                 active[rid][2] = left - n
             for kind, rids in ready.items():
                 if kind in busy:
@@ -184,11 +214,13 @@ class Server:
                     r, sol, left, fp = active[rids[0]]
                     n = min(self.chunk, left)
                     sol.step(n)
+                    watch(sol)
                     active[rids[0]][2] = left - n
                     continue
                 n = min([self.chunk] + [active[rid][2] for rid in rids])
                 g = Group([active[rid][1] for rid in rids])
                 g.step(n)
+                watch(g)
                 groups.append(g)
                 grouped[id(g)] = rids
                 for rid in rids:
@@ -197,10 +229,9 @@ class Server:
                 self.groups_formed += 1
                 self.grouped_request_chunks += len(rids)
             if not progressed:
-                if not active and pending and not waiting:
-                    time.sleep(max(0.0, min(pending[0].arrival - clock(), 0.01)))
-                else:
-                    time.sleep(50e-6)
+                # until a chunk finishes or the next request arrives
+                wait = pending[0].arrival - clock() if pending else 0.01
+                wake.wait(max(0.0, min(wait, 0.01)))
         makespan = clock()
         return summarize(requests, makespan, peak_bytes=self.peak_bytes, peak_active=self.peak_active,
                          groups_formed=self.groups_formed, grouped_request_chunks=self.grouped_request_chunks)
