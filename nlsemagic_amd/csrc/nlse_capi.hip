@@ -461,10 +461,10 @@ long long nlse_footprint_bytes(int dim, int N, int M, int L, int method, int pre
 {
     const long long n = (long long)N * (dim >= 2 ? M : 1) * (dim == 3 ? L : 1);
     const long long e = precision == 2 ? 16 : 8;
-    (void)method;
     long long b = 4 * n * e + n * (e / 2);  // U, Y0, Y1, ktot, V
     if (dim == 3 && method == 2) b += n * e;
     if (precision == 3) b += n * e;  // low part of the compensated state
+    b += 8LL * solver_ws_len(N, M, L);  // reduction workspace
     return b;
 }
 
