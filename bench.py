@@ -663,13 +663,24 @@ def run_serving(args):
         same = sum(int(np.array_equal(served[b.rid].view(np.uint64), b.result.view(np.uint64)))
                    for b in base if b.precision == 2)
         n64 = sum(1 for b in base if b.precision == 2)
+        # each request's service time alone (FP64 drop-in: state built, steps,
+        # download) against its latency in the server: the tail is set by the
+        # largest requests' own compute when the ratio stays near 1
+        alone = {b.rid: b.finished - b.admitted for b in base}
+        worst = max(reqs, key=lambda r: r.latency)
+        ratio64 = [r.latency / alone[r.rid] for r in reqs if r.precision == 2]
         points[qps] = {"server": stats, "one_at_a_time_fp64": plain,
                        "latency_p50_s": stats["latency_p50_s"], "latency_p99_s": stats["latency_p99_s"],
                        "latency_p50_by_precision": {("fp64" if p == 2 else "fp32c"): float(np.percentile(v, 50))
                                                     for p, v in lat_tagged.items() if v},
                        "msd_requests": sum(1 for r in reqs if r.BC == 2),
                        "trap_requests": sum(1 for r in reqs if r.make_V is not None),
-                       "fp64_requests_identical_to_one_at_a_time": f"{same}/{n64}"}
+                       "fp64_requests_identical_to_one_at_a_time": f"{same}/{n64}",
+                       "slowest_request": {"rid": worst.rid, "shape": list(worst.shape), "method": worst.method,
+                                           "precision": worst.precision, "steps": worst.steps,
+                                           "latency_s": worst.latency, "alone_fp64_s": alone[worst.rid]},
+                       "fp64_latency_over_alone_p50_p99": [float(np.percentile(ratio64, 50)),
+                                                           float(np.percentile(ratio64, 99))]}
         print(f"serving at {qps} QPS: p50 {stats['latency_p50_s']:.3f} s, p99 {stats['latency_p99_s']:.3f} s",
               file=sys.stderr, flush=True)
     q0 = next(iter(points))

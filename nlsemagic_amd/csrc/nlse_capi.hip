@@ -6,10 +6,12 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "nlse_host.h"
@@ -33,6 +35,170 @@ hipError_t dev_malloc_raw(void** p, size_t bytes)
 {
     ++g_allocs;
     return hipMalloc(p, bytes);
+}
+
+namespace {
+
+// host threads that split one memcpy between them
+class CopyPool {
+public:
+    explicit CopyPool(int n) : n_(n)
+    {
+        for (int i = 0; i < n; ++i) th_.emplace_back([this, i] { run(i); });
+    }
+    // the calling thread copies the last of n + 1 parts
+    void copy(void* dst, const void* src, size_t len)
+    {
+        std::unique_lock<std::mutex> lk(m_);
+        dst_ = (char*)dst;
+        src_ = (const char*)src;
+        len_ = len;
+        pending_ = n_;
+        ++gen_;
+        go_.notify_all();
+        lk.unlock();
+        part(n_, (char*)dst, (const char*)src, len);
+        lk.lock();
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    void part(int id, char* dst, const char* src, size_t len) const
+    {
+        const size_t per = ((len + n_) / (n_ + 1) + 63) / 64 * 64;  // whole cache lines per part
+        const size_t a = std::min(len, per * id), b = std::min(len, a + per);
+        if (b > a) std::memcpy(dst + a, src + a, b - a);
+    }
+    void run(int id)
+    {
+        unsigned seen = 0;
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            go_.wait(lk, [&] { return gen_ != seen; });
+            seen = gen_;
+            char* d = dst_;
+            const char* sp = src_;
+            const size_t len = len_;
+            lk.unlock();
+            part(id, d, sp, len);
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    int n_;
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable go_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t len_ = 0;
+    int pending_ = 0;
+    unsigned gen_ = 0;
+};
+
+// pinned chunks of the pageable copies, shared by every solver of the
+// process (one copy at a time); created on the first large copy and kept to
+// the end of the process, with the pool's threads
+#ifndef NLSE_XFER_CHUNK_MB
+#define NLSE_XFER_CHUNK_MB 4
+#endif
+#ifndef NLSE_XFER_SLOTS
+#define NLSE_XFER_SLOTS 3
+#endif
+#ifndef NLSE_XFER_THREADS
+#define NLSE_XFER_THREADS 8
+#endif
+constexpr size_t XFER_CHUNK = (size_t)NLSE_XFER_CHUNK_MB << 20;  // bytes per pinned chunk
+constexpr int XFER_SLOTS = NLSE_XFER_SLOTS;
+constexpr size_t XFER_DIRECT = 1u << 20;  // smaller copies: one plain hipMemcpyAsync
+struct Staging {
+    std::mutex m;
+    bool tried = false, ok = false;
+    char* pin[XFER_SLOTS] = {};
+    hipEvent_t ev[XFER_SLOTS] = {};
+    bool used[XFER_SLOTS] = {};
+    CopyPool* pool = nullptr;
+};
+
+Staging& staging()
+{
+    static Staging* s = new Staging();
+    return *s;
+}
+
+// under S.m
+bool staging_ready(Staging& S)
+{
+    if (S.tried) return S.ok;
+    S.tried = true;
+    for (int i = 0; i < XFER_SLOTS; ++i) {
+        if (hipHostMalloc((void**)&S.pin[i], XFER_CHUNK, hipHostMallocDefault) != hipSuccess) return false;
+        if (hipEventCreateWithFlags(&S.ev[i], hipEventDisableTiming) != hipSuccess) return false;
+    }
+    const unsigned hc = std::thread::hardware_concurrency();
+    // pool threads besides the caller: at most NLSE_XFER_THREADS - 1, half the cores
+    S.pool = new CopyPool((int)std::max(1u, std::min((unsigned)NLSE_XFER_THREADS - 1, hc / 2)));
+    S.ok = true;
+    return true;
+}
+
+}  // namespace
+
+int h2d_pageable(void* dst, const void* src, size_t bytes, hipStream_t st)
+{
+    Staging& S = staging();
+    std::lock_guard<std::mutex> lk(S.m);
+    if (bytes < XFER_DIRECT || !staging_ready(S)) {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        return NLSE_OK;
+    }
+    int sl = 0;
+    for (size_t off = 0; off < bytes; off += XFER_CHUNK, sl = (sl + 1) % XFER_SLOTS) {
+        const size_t len = std::min(XFER_CHUNK, bytes - off);
+        if (S.used[sl]) HIPCHK(hipEventSynchronize(S.ev[sl]));  // its previous copy has left the chunk
+        S.pool->copy(S.pin[sl], (const char*)src + off, len);
+        HIPCHK(hipMemcpyAsync((char*)dst + off, S.pin[sl], len, hipMemcpyHostToDevice, st));
+        HIPCHK(hipEventRecord(S.ev[sl], st));
+        S.used[sl] = true;
+    }
+    return NLSE_OK;
+}
+
+int d2h_pageable(void* dst, const void* src, size_t bytes, hipStream_t st)
+{
+    Staging& S = staging();
+    std::lock_guard<std::mutex> lk(S.m);
+    if (bytes < XFER_DIRECT || !staging_ready(S)) {
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return NLSE_OK;
+    }
+    const size_t nch = (bytes + XFER_CHUNK - 1) / XFER_CHUNK;
+    // the DMA of chunk c into slot c mod XFER_SLOTS
+    auto issue = [&](size_t c) -> int {
+        const int sl = (int)(c % XFER_SLOTS);
+        if (S.used[sl]) HIPCHK(hipEventSynchronize(S.ev[sl]));
+        const size_t off = c * XFER_CHUNK, len = std::min(XFER_CHUNK, bytes - off);
+        HIPCHK(hipMemcpyAsync(S.pin[sl], (const char*)src + off, len, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipEventRecord(S.ev[sl], st));
+        S.used[sl] = true;
+        return NLSE_OK;
+    };
+    for (size_t c = 0; c < nch && c < (size_t)XFER_SLOTS; ++c) {
+        const int rc = issue(c);
+        if (rc != NLSE_OK) return rc;
+    }
+    for (size_t c = 0; c < nch; ++c) {
+        const int sl = (int)(c % XFER_SLOTS);
+        HIPCHK(hipEventSynchronize(S.ev[sl]));
+        const size_t off = c * XFER_CHUNK, len = std::min(XFER_CHUNK, bytes - off);
+        S.pool->copy((char*)dst + off, S.pin[sl], len);
+        if (c + XFER_SLOTS < nch) {
+            const int rc = issue(c + XFER_SLOTS);
+            if (rc != NLSE_OK) return rc;
+        }
+    }
+    return NLSE_OK;
 }
 
 void split_hilo(const double* x, float* hi, float* lo, long n)
@@ -314,7 +480,10 @@ int step_impl(nlse_solver* h, int steps)
 template <typename T>
 int upload(nlse_solver* h, const T* Ur, const T* Ui, const T* V)
 {
-    if (V) HIPCHK(hipMemcpyAsync(h->V, V, sizeof(T) * (size_t)h->n, hipMemcpyHostToDevice, h->stream));
+    if (V) {
+        const int rc = h2d_pageable(h->V, V, sizeof(T) * (size_t)h->n, h->stream);
+        if (rc != NLSE_OK) return rc;
+    }
     int rc = upload_planes<T>(Ur, Ui, (c2<T>*)h->U, (T*)h->Y0, h->n, h->stream);
     if (rc == NLSE_OK && h->Ulo) HIPCHK(hipMemsetAsync(h->Ulo, 0, sizeof(c2<T>) * (size_t)h->n, h->stream));
     if (rc == NLSE_OK) HIPCHK(hipStreamSynchronize(h->stream));
@@ -338,7 +507,8 @@ int upload_comp(nlse_solver* h, const double* Ur, const double* Ui, const double
     if (V) {
         std::vector<float> v(n);
         for (long p = 0; p < n; ++p) v[p] = (float)V[p];
-        HIPCHK(hipMemcpyAsync(h->V, v.data(), sizeof(float) * (size_t)n, hipMemcpyHostToDevice, h->stream));
+        const int rv = h2d_pageable(h->V, v.data(), sizeof(float) * (size_t)n, h->stream);
+        if (rv != NLSE_OK) return rv;
         HIPCHK(hipStreamSynchronize(h->stream));
     }
     int rc = upload_planes<float>(hr.data(), hi.data(), (float2*)h->U, (float*)h->Y0, n, h->stream);

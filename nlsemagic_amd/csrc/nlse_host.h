@@ -103,14 +103,26 @@ template <typename T> __global__ void unpack_kernel(const c2<T>* __restrict__ u,
     }
 }
 
+// Copies between pageable host memory (the caller's planes) and the device,
+// enqueued on `st`: large ones go through pinned chunks that a pool of host
+// threads fills (or empties) while the DMA engine moves the previous chunk
+// (nlse_capi.hip).  h2d returns once the source has been read (the copy may
+// still be in flight on st); d2h returns once the data is in `dst`.
+int h2d_pageable(void* dst, const void* src, size_t bytes, hipStream_t st);
+int d2h_pageable(void* dst, const void* src, size_t bytes, hipStream_t st);
+
 // host planes -> interleaved device array `dst` of n points, through the
 // device staging area `stage` (>= 2n elements of T); Ui may be null (zero)
 template <typename T>
 int upload_planes(const T* Ur, const T* Ui, c2<T>* dst, T* stage, long n, hipStream_t st)
 {
     const size_t b = sizeof(T) * (size_t)n;
-    HIPCHK(hipMemcpyAsync(stage, Ur, b, hipMemcpyHostToDevice, st));
-    if (Ui) HIPCHK(hipMemcpyAsync(stage + n, Ui, b, hipMemcpyHostToDevice, st));
+    int rc = h2d_pageable(stage, Ur, b, st);
+    if (rc != NLSE_OK) return rc;
+    if (Ui) {
+        rc = h2d_pageable(stage + n, Ui, b, st);
+        if (rc != NLSE_OK) return rc;
+    }
     else HIPCHK(hipMemsetAsync(stage + n, 0, b, st));
     hipLaunchKernelGGL((pack_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, stage, stage + n, dst, n);
     HIPCHK(hipGetLastError());
@@ -124,8 +136,9 @@ int download_planes(const c2<T>* src, T* Ur, T* Ui, T* stage, long n, hipStream_
     const size_t b = sizeof(T) * (size_t)n;
     hipLaunchKernelGGL((unpack_kernel<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, stage, stage + n, n);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(Ur, stage, b, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(Ui, stage + n, b, hipMemcpyDeviceToHost, st));
+    int rc = d2h_pageable(Ur, stage, b, st);
+    if (rc == NLSE_OK) rc = d2h_pageable(Ui, stage + n, b, st);
+    if (rc != NLSE_OK) return rc;
     HIPCHK(hipStreamSynchronize(st));
     return NLSE_OK;
 }
