@@ -96,9 +96,9 @@ private:
     unsigned gen_ = 0;
 };
 
-// pinned chunks of the pageable copies, shared by every solver of the
-// process (one copy at a time); created on the first large copy and kept to
-// the end of the process, with the pool's threads
+// pinned chunks of the pageable copies, one set per device, shared by every
+// solver on it (one copy at a time); created on the device's first large
+// copy and kept to the end of the process, with the pool's threads
 #ifndef NLSE_XFER_CHUNK_MB
 #define NLSE_XFER_CHUNK_MB 4
 #endif
@@ -120,21 +120,33 @@ struct Staging {
     CopyPool* pool = nullptr;
 };
 
-Staging& staging()
+constexpr int XFER_MAX_DEVICES = 64;
+
+// the staging of the device `st` belongs to (its events must be that device's)
+Staging* staging(hipStream_t st, int& dev)
 {
-    static Staging* s = new Staging();
-    return *s;
+    static std::mutex mu;
+    static Staging* per[XFER_MAX_DEVICES] = {};
+    dev = 0;
+    if (hipStreamGetDevice(st, &dev) != hipSuccess || dev < 0 || dev >= XFER_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!per[dev]) per[dev] = new Staging();
+    return per[dev];
 }
 
-// under S.m
-bool staging_ready(Staging& S)
+// under S.m; events are created on `dev`
+bool staging_ready(Staging& S, int dev)
 {
     if (S.tried) return S.ok;
     S.tried = true;
-    for (int i = 0; i < XFER_SLOTS; ++i) {
-        if (hipHostMalloc((void**)&S.pin[i], XFER_CHUNK, hipHostMallocDefault) != hipSuccess) return false;
-        if (hipEventCreateWithFlags(&S.ev[i], hipEventDisableTiming) != hipSuccess) return false;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(dev) != hipSuccess) return false;
+    bool ok = true;
+    for (int i = 0; i < XFER_SLOTS && ok; ++i) {
+        ok = hipHostMalloc((void**)&S.pin[i], XFER_CHUNK, hipHostMallocDefault) == hipSuccess &&
+             hipEventCreateWithFlags(&S.ev[i], hipEventDisableTiming) == hipSuccess;
     }
+    if (hipSetDevice(cur) != hipSuccess || !ok) return false;
     const unsigned hc = std::thread::hardware_concurrency();
     // pool threads besides the caller: at most NLSE_XFER_THREADS - 1, half the cores
     S.pool = new CopyPool((int)std::max(1u, std::min((unsigned)NLSE_XFER_THREADS - 1, hc / 2)));
@@ -146,12 +158,15 @@ bool staging_ready(Staging& S)
 
 int h2d_pageable(void* dst, const void* src, size_t bytes, hipStream_t st)
 {
-    Staging& S = staging();
-    std::lock_guard<std::mutex> lk(S.m);
-    if (bytes < XFER_DIRECT || !staging_ready(S)) {
+    int dev = 0;
+    Staging* Sp = bytes < XFER_DIRECT ? nullptr : staging(st, dev);
+    std::unique_lock<std::mutex> lk;
+    if (Sp) lk = std::unique_lock<std::mutex>(Sp->m);
+    if (!Sp || !staging_ready(*Sp, dev)) {
         HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
         return NLSE_OK;
     }
+    Staging& S = *Sp;
     int sl = 0;
     for (size_t off = 0; off < bytes; off += XFER_CHUNK, sl = (sl + 1) % XFER_SLOTS) {
         const size_t len = std::min(XFER_CHUNK, bytes - off);
@@ -166,13 +181,16 @@ int h2d_pageable(void* dst, const void* src, size_t bytes, hipStream_t st)
 
 int d2h_pageable(void* dst, const void* src, size_t bytes, hipStream_t st)
 {
-    Staging& S = staging();
-    std::lock_guard<std::mutex> lk(S.m);
-    if (bytes < XFER_DIRECT || !staging_ready(S)) {
+    int dev = 0;
+    Staging* Sp = bytes < XFER_DIRECT ? nullptr : staging(st, dev);
+    std::unique_lock<std::mutex> lk;
+    if (Sp) lk = std::unique_lock<std::mutex>(Sp->m);
+    if (!Sp || !staging_ready(*Sp, dev)) {
         HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         return NLSE_OK;
     }
+    Staging& S = *Sp;
     const size_t nch = (bytes + XFER_CHUNK - 1) / XFER_CHUNK;
     // the DMA of chunk c into slot c mod XFER_SLOTS
     auto issue = [&](size_t c) -> int {
