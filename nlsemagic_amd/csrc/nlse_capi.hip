@@ -111,12 +111,20 @@ private:
 constexpr size_t XFER_CHUNK = (size_t)NLSE_XFER_CHUNK_MB << 20;  // bytes per pinned chunk
 constexpr int XFER_SLOTS = NLSE_XFER_SLOTS;
 constexpr size_t XFER_DIRECT = 1u << 20;  // smaller copies: one plain hipMemcpyAsync
+// The DMAs run on a copy stream of the staging's own, which lives as long
+// as the process: every event the host waits on is recorded there, never on
+// a caller's stream (which may be destroyed, or captured, by the time a
+// later copy reuses the chunk).  The caller's stream and the copy stream
+// are ordered by event waits both ways.
 struct Staging {
     std::mutex m;
     bool tried = false, ok = false;
     char* pin[XFER_SLOTS] = {};
     hipEvent_t ev[XFER_SLOTS] = {};
     bool used[XFER_SLOTS] = {};
+    hipStream_t xs = nullptr;             // copy stream
+    hipEvent_t after = nullptr;           // on the caller's stream: its earlier work
+    hipEvent_t done = nullptr;            // on xs: the copy
     CopyPool* pool = nullptr;
 };
 
@@ -141,7 +149,9 @@ bool staging_ready(Staging& S, int dev)
     S.tried = true;
     int cur = 0;
     if (hipGetDevice(&cur) != hipSuccess || hipSetDevice(dev) != hipSuccess) return false;
-    bool ok = true;
+    bool ok = hipStreamCreateWithFlags(&S.xs, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&S.after, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&S.done, hipEventDisableTiming) == hipSuccess;
     for (int i = 0; i < XFER_SLOTS && ok; ++i) {
         ok = hipHostMalloc((void**)&S.pin[i], XFER_CHUNK, hipHostMallocDefault) == hipSuccess &&
              hipEventCreateWithFlags(&S.ev[i], hipEventDisableTiming) == hipSuccess;
@@ -167,15 +177,21 @@ int h2d_pageable(void* dst, const void* src, size_t bytes, hipStream_t st)
         return NLSE_OK;
     }
     Staging& S = *Sp;
+    // the copy writes dst after the caller's earlier work on st, and st's
+    // later work reads it after the copy
+    HIPCHK(hipEventRecord(S.after, st));
+    HIPCHK(hipStreamWaitEvent(S.xs, S.after, 0));
     int sl = 0;
     for (size_t off = 0; off < bytes; off += XFER_CHUNK, sl = (sl + 1) % XFER_SLOTS) {
         const size_t len = std::min(XFER_CHUNK, bytes - off);
         if (S.used[sl]) HIPCHK(hipEventSynchronize(S.ev[sl]));  // its previous copy has left the chunk
         S.pool->copy(S.pin[sl], (const char*)src + off, len);
-        HIPCHK(hipMemcpyAsync((char*)dst + off, S.pin[sl], len, hipMemcpyHostToDevice, st));
-        HIPCHK(hipEventRecord(S.ev[sl], st));
+        HIPCHK(hipMemcpyAsync((char*)dst + off, S.pin[sl], len, hipMemcpyHostToDevice, S.xs));
+        HIPCHK(hipEventRecord(S.ev[sl], S.xs));
         S.used[sl] = true;
     }
+    HIPCHK(hipEventRecord(S.done, S.xs));
+    HIPCHK(hipStreamWaitEvent(st, S.done, 0));
     return NLSE_OK;
 }
 
@@ -191,14 +207,18 @@ int d2h_pageable(void* dst, const void* src, size_t bytes, hipStream_t st)
         return NLSE_OK;
     }
     Staging& S = *Sp;
+    // the copy reads src after the caller's earlier work on st (it has
+    // finished, chunk by chunk, when this returns)
+    HIPCHK(hipEventRecord(S.after, st));
+    HIPCHK(hipStreamWaitEvent(S.xs, S.after, 0));
     const size_t nch = (bytes + XFER_CHUNK - 1) / XFER_CHUNK;
     // the DMA of chunk c into slot c mod XFER_SLOTS
     auto issue = [&](size_t c) -> int {
         const int sl = (int)(c % XFER_SLOTS);
         if (S.used[sl]) HIPCHK(hipEventSynchronize(S.ev[sl]));
         const size_t off = c * XFER_CHUNK, len = std::min(XFER_CHUNK, bytes - off);
-        HIPCHK(hipMemcpyAsync(S.pin[sl], (const char*)src + off, len, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipEventRecord(S.ev[sl], st));
+        HIPCHK(hipMemcpyAsync(S.pin[sl], (const char*)src + off, len, hipMemcpyDeviceToHost, S.xs));
+        HIPCHK(hipEventRecord(S.ev[sl], S.xs));
         S.used[sl] = true;
         return NLSE_OK;
     };
