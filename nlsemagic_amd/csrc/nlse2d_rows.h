@@ -281,9 +281,7 @@ __device__ __forceinline__ void rows2d_tile(RowsLds<T, METHOD, FIRST, LAST, TI>&
                         // in any case (single precision CD 7-8 % faster on
                         // 1024^2; FP64 and the compensated state gain
                         // nothing, and keep plain stores)
-                        const c2<T> r = mk<T>(uo.x + c.k6 * a.x, uo.y + c.k6 * a.y);
-                        __builtin_nontemporal_store(r.x, &out[p].x);
-                        __builtin_nontemporal_store(r.y, &out[p].y);
+                        st_stream<T>(&out[p], mk<T>(uo.x + c.k6 * a.x, uo.y + c.k6 * a.y));
                     } else {
                         out[p] = mk<T>(uo.x + c.k6 * a.x, uo.y + c.k6 * a.y);
                     }

@@ -212,8 +212,8 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
                         const c2<T> b = Yr[a2][r];
                         a.x = a.x + (T)2.0 * f.x;
                         a.y = a.y + (T)2.0 * f.y;
-                        acc[pp] = a;
-                        yout[pp] = mk<T>(b.x + c.k2 * f.x, b.y + c.k2 * f.y);
+                        st_stream<T>(&acc[pp], a);
+                        st_stream<T>(&yout[pp], mk<T>(b.x + c.k2 * f.x, b.y + c.k2 * f.y));
                     } else {
                         // stage 4: U + k/6 (ktot + K4), compensated when a low part is kept
                         c2<T> b = Br[a2][r];
@@ -223,10 +223,10 @@ __device__ __forceinline__ void cdpair_tile(CdPairLds<T, TJ>& sm, int i0, int j0
                             c2<T> lo = Ln[r];
                             comp_add<T>(b.x, lo.x, c.k6 * a.x);
                             comp_add<T>(b.y, lo.y, c.k6 * a.y);
-                            ulo[pp] = lo;
-                            yout[pp] = b;
+                            st_stream<T>(&ulo[pp], lo);
+                            st_stream<T>(&yout[pp], b);
                         } else {
-                            yout[pp] = mk<T>(b.x + c.k6 * a.x, b.y + c.k6 * a.y);
+                            st_stream<T>(&yout[pp], mk<T>(b.x + c.k6 * a.x, b.y + c.k6 * a.y));
                         }
                     }
                 }

@@ -352,8 +352,8 @@ __device__ __forceinline__ void shocpair_tile(ShocPairLds<T, TJ>& sm, int i0, in
                                 // stage 2: ktot = K1 + 2 K2, Y2 = U + k/2 K2
                                 a.x = a.x + (T)2.0 * f.x;
                                 a.y = a.y + (T)2.0 * f.y;
-                                acc[pp] = a;
-                                yout[pp] = mk<T>(b.x + c.k2 * f.x, b.y + c.k2 * f.y);
+                                st_stream<T>(&acc[pp], a);
+                                st_stream<T>(&yout[pp], mk<T>(b.x + c.k2 * f.x, b.y + c.k2 * f.y));
                             } else {
                                 // stage 4: U + k/6 (ktot + K4), compensated when a low part is kept
                                 a.x = a.x + (T)1.0 * f.x;
@@ -362,10 +362,10 @@ __device__ __forceinline__ void shocpair_tile(ShocPairLds<T, TJ>& sm, int i0, in
                                     c2<T> lo = Lb[r];
                                     comp_add<T>(b.x, lo.x, c.k6 * a.x);
                                     comp_add<T>(b.y, lo.y, c.k6 * a.y);
-                                    ulo[pp] = lo;
-                                    yout[pp] = b;
+                                    st_stream<T>(&ulo[pp], lo);
+                                    st_stream<T>(&yout[pp], b);
                                 } else {
-                                    yout[pp] = mk<T>(b.x + c.k6 * a.x, b.y + c.k6 * a.y);
+                                    st_stream<T>(&yout[pp], mk<T>(b.x + c.k6 * a.x, b.y + c.k6 * a.y));
                                 }
                             }
                         }

@@ -19,6 +19,14 @@ template <> struct C2<double> { using type = double2; };
 template <> struct C2<float> { using type = float2; };
 template <typename T> using c2 = typename C2<T>::type;
 
+// store with a streaming (nontemporal) hint: for a kernel's outputs that the
+// next launch reads after the kernel boundary, from the Infinity Cache
+template <typename T> __device__ __forceinline__ void st_stream(c2<T>* p, c2<T> v)
+{
+    __builtin_nontemporal_store(v.x, &p->x);
+    __builtin_nontemporal_store(v.y, &p->y);
+}
+
 template <typename T> __host__ __device__ inline c2<T> mk(T x, T y) { c2<T> r; r.x = x; r.y = y; return r; }
 
 // f(integral_constant<int, B>), ..., f(integral_constant<int, E-1>): a loop
